@@ -1,0 +1,172 @@
+/*
+ * lbm_mi355x.h -- C ABI of the MI355X (gfx950) stream-collide library.
+ *
+ * The reference has no plugin API: its seam is inside main(), where the
+ * device globals f_mem_d / mv_mem_d / solids_mem_d (src/lbgpu_prime.cu:17-24),
+ * the __constant__ parameter block (src/lattice.c:2222-2268, uploaded in
+ * src/latman.c:147-376) and the <<<>>> launches of the time loop
+ * (src/lbgpu_prime.cu:369-915) meet the host code.  Each entry point below
+ * replaces one piece of that seam; the comment names the piece.
+ *
+ * Contract (mirrors the reference's error convention, src/lattice.c:3123-3142):
+ *   - every function returns 0 on success and nonzero on failure, with the
+ *     message in lbm_last_error(); the host maps nonzero to process_exit(1);
+ *   - every call is synchronous at the API boundary (one HIP stream inside);
+ *   - a handle is not thread-safe.
+ * Population arrays at the boundary are FP64, SoA [Q][N] with N = LX*LY*LZ
+ * nodes, i fastest, in the reference's direction order (opposite directions
+ * adjacent: src/lbgpu_prime.h:78-84) and hold the reference's slot contents
+ * (shifted flow populations f - w rho_A; plain solute populations).
+ */
+#ifndef LBM_MI355X_H
+#define LBM_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lbm_handle lbm_handle;
+
+enum lbm_storage { LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16 = 2 };
+enum lbm_compute { LBM_COMPUTE_F64 = 0, LBM_COMPUTE_F32 = 1 };
+
+/* Parameters: the fields read_params() fills for the hot path
+ * (src/params.h:40-661, src/lattice.h:113-300) plus storage precision and
+ * the z-face boundary conditions, which have no reference counterpart. */
+typedef struct lbm_params {
+  int dims;            /* NUM_DIMENSIONS: 2 (D2Q9) or 3 (D3Q19) */
+  int ni, nj, nk;      /* LX, LY, LZ (nk = 1 in 2-D) */
+  int nsubs;           /* NUM_FLUID_COMPONENTS: 1 flow, 2 flow + solute */
+  int halo_mode;       /* 0: periodic wrap in every axis (serial build);
+                          1: the last axis crosses the slab faces through
+                             halo planes (PARALLEL build, one z-slab) */
+  double tau[2];
+  double rho_A[2];
+  double gaccel[6];    /* [subs*3 + component] */
+  double sink;
+  int sigma_t_on, sigma_t_off;
+  int pressure_n_in[2], pressure_n_out[2], pressure_s_in[2], pressure_s_out[2];
+  int zeroconcgrad_n, zeroconcgrad_s;
+  double rho_in, rho_out;
+  double pfadd, pfmul;
+  int bc_ramp_start, bc_ramp_stop;
+  const double *pressure_n_series[2];
+  int pressure_n_series_len[2];
+  const double *pressure_s_series[2];
+  int pressure_s_series_len[2];
+  int zface_pressure;  /* 3-D Zou-He: rho_bot at z=0, rho_top at z=nk-1 */
+  double rho_bot, rho_top;
+  int zface_solute;    /* 3-D solute: Dirichlet c at z=0, zero gradient at z=nk-1 */
+  double c_bot;
+  const double *c_bot_map; /* optional [nj][ni] inlet concentration */
+  int flux_plane;      /* plane (last axis) whose solute moments are summed
+                          every step; -1 = none */
+  int storage;         /* enum lbm_storage */
+  int compute;         /* enum lbm_compute */
+  double store_scale[2]; /* FP16: stored = (value - shift) * scale; 0 = default */
+  double store_shift_c;  /* solute storage shift c0: stored g - w_a c0 */
+} lbm_params;
+
+/* reductions over the fluid nodes of the last macroscopic write */
+enum lbm_reduction {
+  LBM_REDUCE_REF = 0,  /* compute.c set for one subs, 12 values:
+                          max|rho|, min|rho|, ave rho, max|u_i| (3),
+                          min|u_i| (3), ave u_i (3)  (src/compute.c:1159-1416) */
+  LBM_REDUCE_SUMS = 1  /* 6 values: sum(rho - rho_A), sum u_eq (3), number of
+                          fluid nodes, sum c (solute) */
+};
+
+const char *lbm_last_error(void);
+
+/* construct_lattice: allocation, constants (src/latman.c:91-493) */
+int lbm_create(const lbm_params *p, lbm_handle **out);
+/* destruct_lattice (src/latman.c:2084-2177) */
+int lbm_destroy(lbm_handle *h);
+
+/* solids H2D with halo copies (src/lbgpu_prime.cu:170-257).  mask: N
+ * bytes, i fastest; a node is solid when its byte equals 1, as on the
+ * reference GPU path (src/lattice.c:2291).  Builds the fluid-node
+ * compaction and the neighbour table; populations become zero. */
+int lbm_set_solids(lbm_handle *h, const uint8_t *mask);
+
+/* f H2D / D2H per subs (src/lbgpu_prime.cu:317-343, :874-900).  Solid
+ * nodes are not stored: download returns 0 there. */
+int lbm_upload_f(lbm_handle *h, int subs, const double *f1d);
+int lbm_download_f(lbm_handle *h, int subs, double *f1d);
+
+/* init_problem for u = 0 (src/latman.c:1786-1893): flow f~ = 0, solute
+ * g = w c0 (c0: [N] or NULL for 0) */
+int lbm_init_rest(lbm_handle *h, const double *c0);
+
+/* time counter of the loop (src/lbgpu_prime.cu:369,601) */
+int lbm_set_time(lbm_handle *h, int time);
+int lbm_get_time(const lbm_handle *h, int *time);
+
+/* One LBM step at time+1: odd times run the halo exchange (halo_mode),
+ * bcs_1 and k_stream_collide_stream (src/lbgpu_prime.cu:382-530); even
+ * times the exchange, bcs_2 and k_collide (:612-752).  write_macro stores
+ * the pre-collision moments (is_end_of_frame_mem_c). */
+int lbm_step(lbm_handle *h, int write_macro);
+/* nsteps steps, macroscopic write on the last one if write_macro_last */
+int lbm_run(lbm_handle *h, int nsteps, int write_macro_last);
+
+/* The two half-steps on their own, at an explicit time (a1 / a2):
+ * k_stream_collide_stream (src/k_stream_collide_stream.c:2-442) and
+ * k_collide (src/k_collide.c:4-434), without exchange or BCs. */
+int lbm_step_odd(lbm_handle *h, int time, int write_macro);
+int lbm_step_even(lbm_handle *h, int time, int write_macro);
+
+/* bcs_1 (phase 1) / bcs_2 (phase 2) at time (src/bcs.c:226-557,1179-1510);
+ * in 3-D the z-face conditions */
+int lbm_apply_bcs(lbm_handle *h, int phase, int time);
+
+/* k_bound_DtH_{1,2} / k_bound_HtD_{1,2} (src/boundary_kernels.c:1-139):
+ * pack the z-face planes into / unpack from DEVICE buffers laid out
+ * [subs][bound pair][LX*LY] in the storage type.  Phase-2 unpack stores
+ * only where the cross-face pushing node is fluid.
+ * lbm_halo_bytes gives the size of one buffer. */
+int lbm_halo_bytes(const lbm_handle *h, long *bytes);
+int lbm_halo_pack(lbm_handle *h, int phase, void *pos_dev, void *neg_dev);
+int lbm_halo_unpack(lbm_handle *h, int phase, const void *pos_dev, const void *neg_dev);
+/* one-slab stand-in for process_send_recv_*: pack, then unpack the own
+ * buffers as if received from both neighbours (periodic wrap) */
+int lbm_halo_periodic(lbm_handle *h, int phase);
+
+/* mv D2H (src/lbgpu_prime.cu:534-583): rho (or c for subs 1) and u_eq per
+ * node; solid nodes get 0; uz may be NULL in 2-D */
+int lbm_download_macro(lbm_handle *h, int subs, double *rho, double *ux, double *uy, double *uz);
+
+/* FP64 reductions on the GPU over the last macroscopic write */
+int lbm_reduce(lbm_handle *h, int kind, int subs, double *out);
+/* solute moments of plane flux_plane in the last step */
+int lbm_last_flux(lbm_handle *h, double *flux, double *mass);
+
+/* geometry numbers: N, fluid nodes, stored slots per direction row */
+int lbm_info(const lbm_handle *h, long *n_nodes, long *n_fluid, long *row_stride);
+
+/* Kernel timing with HIP events on the library's stream.  While enabled
+ * every launch is bracketed by events; lbm_profile_read returns the summed
+ * milliseconds and launch counts of the kernel classes
+ * [0] odd half-step, [1] even half-step, [2] boundary conditions,
+ * [3] halo pack/unpack, [4] plane flux, and resets them. */
+int lbm_profile_enable(lbm_handle *h, int on);
+int lbm_profile_read(lbm_handle *h, double *ms, int *count);
+
+/* Seeded synthetic porous media (xoshiro256** seeded by splitmix64).
+ * 2-D: non-overlapping discs, radius U[rmin,rmax], periodic in x, solid
+ * rows j=0 and j=nj-1 when walls; discs may not touch the walls.
+ * 3-D: overlapping spheres, periodic in x and y, no solid within zbuf
+ * planes of either z end.  Filled until the porosity is <= phi.
+ * mask gets 1 for solid, 0 for fluid.  Returns the achieved porosity in
+ * *phi_out. */
+int lbm_geom_discs2d(int ni, int nj, uint64_t seed, double rmin, double rmax, double phi,
+                     int walls, uint8_t *mask, double *phi_out);
+int lbm_geom_spheres3d(int ni, int nj, int nk, uint64_t seed, double rmin, double rmax,
+                       double phi, int zbuf, uint8_t *mask, double *phi_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

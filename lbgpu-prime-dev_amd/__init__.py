@@ -1,0 +1,316 @@
+"""Python host layer over the MI355X stream-collide C ABI
+(include/lbm_mi355x.h, lib/liblbm_mi355x.so).
+
+It mirrors the host side of the reference's time loop (src/lbgpu_prime.cu:
+construct_lattice, the odd/even half-steps with bcs_1/bcs_2, the frame
+transfers and the compute.c reductions) for callers and tests written in
+Python.  Every compute call goes to the HIP library; there is no CPU path:
+if the library is not built this module raises on import.
+
+The directory name is not a Python identifier; load it with
+``load_package()`` from the repository root or importlib.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "liblbm_mi355x.so")
+
+STORE_F64, STORE_F32, STORE_F16 = 0, 1, 2
+COMPUTE_F64, COMPUTE_F32 = 0, 1
+REDUCE_REF, REDUCE_SUMS = 0, 1
+PROFILE_CLASSES = ("odd", "even", "bc", "halo", "flux")
+
+
+class LbmError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("dims", ctypes.c_int), ("ni", ctypes.c_int), ("nj", ctypes.c_int), ("nk", ctypes.c_int),
+        ("nsubs", ctypes.c_int), ("halo_mode", ctypes.c_int),
+        ("tau", ctypes.c_double * 2), ("rho_A", ctypes.c_double * 2),
+        ("gaccel", ctypes.c_double * 6), ("sink", ctypes.c_double),
+        ("sigma_t_on", ctypes.c_int), ("sigma_t_off", ctypes.c_int),
+        ("pressure_n_in", ctypes.c_int * 2), ("pressure_n_out", ctypes.c_int * 2),
+        ("pressure_s_in", ctypes.c_int * 2), ("pressure_s_out", ctypes.c_int * 2),
+        ("zeroconcgrad_n", ctypes.c_int), ("zeroconcgrad_s", ctypes.c_int),
+        ("rho_in", ctypes.c_double), ("rho_out", ctypes.c_double),
+        ("pfadd", ctypes.c_double), ("pfmul", ctypes.c_double),
+        ("bc_ramp_start", ctypes.c_int), ("bc_ramp_stop", ctypes.c_int),
+        ("pressure_n_series", ctypes.POINTER(ctypes.c_double) * 2),
+        ("pressure_n_series_len", ctypes.c_int * 2),
+        ("pressure_s_series", ctypes.POINTER(ctypes.c_double) * 2),
+        ("pressure_s_series_len", ctypes.c_int * 2),
+        ("zface_pressure", ctypes.c_int), ("rho_bot", ctypes.c_double), ("rho_top", ctypes.c_double),
+        ("zface_solute", ctypes.c_int), ("c_bot", ctypes.c_double),
+        ("c_bot_map", ctypes.POINTER(ctypes.c_double)),
+        ("flux_plane", ctypes.c_int),
+        ("storage", ctypes.c_int), ("compute", ctypes.c_int),
+        ("store_scale", ctypes.c_double * 2), ("store_shift_c", ctypes.c_double),
+    ]
+
+
+_LIB = None
+
+
+def lib():
+    """The loaded C ABI library (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise LbmError(f"HIP library missing: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+        u8p = ctypes.POINTER(ctypes.c_uint8)
+        lp = ctypes.POINTER(ctypes.c_long)
+        sig = {
+            "lbm_create": [ctypes.POINTER(Params), ctypes.POINTER(vp)],
+            "lbm_destroy": [vp],
+            "lbm_set_solids": [vp, u8p],
+            "lbm_upload_f": [vp, ip, dp],
+            "lbm_download_f": [vp, ip, dp],
+            "lbm_init_rest": [vp, dp],
+            "lbm_set_time": [vp, ip],
+            "lbm_get_time": [vp, ctypes.POINTER(ctypes.c_int)],
+            "lbm_step": [vp, ip],
+            "lbm_run": [vp, ip, ip],
+            "lbm_step_odd": [vp, ip, ip],
+            "lbm_step_even": [vp, ip, ip],
+            "lbm_apply_bcs": [vp, ip, ip],
+            "lbm_halo_bytes": [vp, lp],
+            "lbm_halo_pack": [vp, ip, vp, vp],
+            "lbm_halo_unpack": [vp, ip, vp, vp],
+            "lbm_halo_periodic": [vp, ip],
+            "lbm_download_macro": [vp, ip, dp, dp, dp, dp],
+            "lbm_reduce": [vp, ip, ip, dp],
+            "lbm_last_flux": [vp, dp, dp],
+            "lbm_info": [vp, lp, lp, lp],
+            "lbm_profile_enable": [vp, ip],
+            "lbm_profile_read": [vp, dp, ctypes.POINTER(ctypes.c_int)],
+            "lbm_geom_discs2d": [ip, ip, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, ip, u8p, dp],
+            "lbm_geom_spheres3d": [ip, ip, ip, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, ip, u8p, dp],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        L.lbm_last_error.restype = ctypes.c_char_p
+        L.lbm_last_error.argtypes = []
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        raise LbmError(lib().lbm_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def make_params(**kw):
+    p = Params()
+    p.dims = kw.get("dims", 3)
+    p.ni, p.nj = kw["ni"], kw["nj"]
+    p.nk = kw.get("nk", 1)
+    p.nsubs = kw.get("nsubs", 1)
+    p.halo_mode = kw.get("halo_mode", 0)
+    tau, rho_A = kw.get("tau", (1.0, 1.0)), kw.get("rho_A", (1.0, 1.0))
+    g = tuple(kw.get("gaccel", (0.0,) * 6)) + (0.0,) * 6
+    for q in range(2):
+        p.tau[q], p.rho_A[q] = tau[q], rho_A[q]
+    for q in range(6):
+        p.gaccel[q] = g[q]
+    p.sink = kw.get("sink", 0.0)
+    p.sigma_t_on = kw.get("sigma_t_on", 0)
+    p.sigma_t_off = kw.get("sigma_t_off", 2**31 - 1)
+    for name in ("pressure_n_in", "pressure_n_out", "pressure_s_in", "pressure_s_out"):
+        v = kw.get(name, (0, 0))
+        arr = getattr(p, name)
+        arr[0], arr[1] = v[0], v[1]
+    p.zeroconcgrad_n = kw.get("zeroconcgrad_n", 0)
+    p.zeroconcgrad_s = kw.get("zeroconcgrad_s", 0)
+    p.rho_in, p.rho_out = kw.get("rho_in", 1.0), kw.get("rho_out", 1.0)
+    p.pfadd, p.pfmul = kw.get("pfadd", 0.0), kw.get("pfmul", 1.0)
+    p.bc_ramp_start, p.bc_ramp_stop = kw.get("bc_ramp_start", -1), kw.get("bc_ramp_stop", -1)
+    p.zface_pressure = kw.get("zface_pressure", 0)
+    p.rho_bot, p.rho_top = kw.get("rho_bot", 1.0), kw.get("rho_top", 1.0)
+    p.zface_solute = kw.get("zface_solute", 0)
+    p.c_bot = kw.get("c_bot", 1.0)
+    p.flux_plane = kw.get("flux_plane", -1)
+    p.storage = kw.get("storage", STORE_F64)
+    p.compute = kw.get("compute", COMPUTE_F64 if p.storage == STORE_F64 else COMPUTE_F32)
+    sc = kw.get("store_scale", (0.0, 0.0))
+    p.store_scale[0], p.store_scale[1] = sc[0], sc[1]
+    p.store_shift_c = kw.get("store_shift_c", 0.0)
+    return p
+
+
+class Lattice:
+    """One lattice on the GPU (lbm_create + lbm_set_solids)."""
+
+    def __init__(self, mask=None, **kw):
+        self.kw = dict(kw)
+        self.p = make_params(**kw)
+        keep = []
+        for name in ("pressure_n_series", "pressure_s_series"):
+            ser = kw.get(name)
+            if ser is not None:
+                for q, s in enumerate(ser):
+                    if s is None:
+                        continue
+                    arr = np.ascontiguousarray(s, dtype=np.float64)
+                    keep.append(arr)
+                    getattr(self.p, name)[q] = _dp(arr)
+                    getattr(self.p, name + "_len")[q] = arr.size
+        cmap = kw.get("c_bot_map")
+        if cmap is not None:
+            arr = np.ascontiguousarray(cmap, dtype=np.float64).ravel()
+            keep.append(arr)
+            self.p.c_bot_map = _dp(arr)
+        self.dims = self.p.dims
+        self.Q = 9 if self.dims == 2 else 19
+        self.N = self.p.ni * self.p.nj * (self.p.nk if self.dims == 3 else 1)
+        self.nsubs = self.p.nsubs
+        h = ctypes.c_void_p()
+        _check(lib().lbm_create(ctypes.byref(self.p), ctypes.byref(h)))
+        self.h = h
+        if mask is not None:
+            self.set_solids(mask)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lbm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        return (self.p.nj, self.p.ni) if self.dims == 2 else (self.p.nk, self.p.nj, self.p.ni)
+
+    def set_solids(self, mask):
+        m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+        if m.size != self.N:
+            raise LbmError("mask size does not match the lattice")
+        _check(lib().lbm_set_solids(self.h, m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+
+    def info(self):
+        n, nf, s = ctypes.c_long(), ctypes.c_long(), ctypes.c_long()
+        _check(lib().lbm_info(self.h, ctypes.byref(n), ctypes.byref(nf), ctypes.byref(s)))
+        return {"nodes": n.value, "fluid": nf.value, "row_stride": s.value}
+
+    def upload_f(self, subs, f):
+        a = np.ascontiguousarray(f, dtype=np.float64).reshape(self.Q, self.N)
+        _check(lib().lbm_upload_f(self.h, subs, _dp(a)))
+
+    def download_f(self, subs):
+        a = np.empty((self.Q, self.N))
+        _check(lib().lbm_download_f(self.h, subs, _dp(a)))
+        return a
+
+    def init_rest(self, c0=None):
+        if c0 is None:
+            _check(lib().lbm_init_rest(self.h, None))
+        else:
+            c = np.ascontiguousarray(c0, dtype=np.float64).ravel()
+            _check(lib().lbm_init_rest(self.h, _dp(c)))
+
+    @property
+    def time(self):
+        t = ctypes.c_int()
+        _check(lib().lbm_get_time(self.h, ctypes.byref(t)))
+        return t.value
+
+    @time.setter
+    def time(self, t):
+        _check(lib().lbm_set_time(self.h, int(t)))
+
+    def step(self, write_mv=False):
+        _check(lib().lbm_step(self.h, int(bool(write_mv))))
+
+    def run(self, n, write_last=True):
+        _check(lib().lbm_run(self.h, int(n), int(bool(write_last))))
+
+    # the reference's pieces of one step pair
+    def stream_collide_stream(self, time, write_mv=False):
+        _check(lib().lbm_step_odd(self.h, int(time), int(bool(write_mv))))
+
+    def collide(self, time, write_mv=False):
+        _check(lib().lbm_step_even(self.h, int(time), int(bool(write_mv))))
+
+    def bcs_1(self, time):
+        _check(lib().lbm_apply_bcs(self.h, 1, int(time)))
+
+    def bcs_2(self, time):
+        _check(lib().lbm_apply_bcs(self.h, 2, int(time)))
+
+    def halo_bytes(self):
+        b = ctypes.c_long()
+        _check(lib().lbm_halo_bytes(self.h, ctypes.byref(b)))
+        return b.value
+
+    def halo_pack(self, phase, pos_ptr, neg_ptr):
+        _check(lib().lbm_halo_pack(self.h, phase, ctypes.c_void_p(pos_ptr), ctypes.c_void_p(neg_ptr)))
+
+    def halo_unpack(self, phase, pos_ptr, neg_ptr):
+        _check(lib().lbm_halo_unpack(self.h, phase, ctypes.c_void_p(pos_ptr), ctypes.c_void_p(neg_ptr)))
+
+    def halo_periodic(self, phase):
+        _check(lib().lbm_halo_periodic(self.h, phase))
+
+    def mv(self, subs=0):
+        rho, ux, uy, uz = (np.empty(self.N) for _ in range(4))
+        _check(lib().lbm_download_macro(self.h, subs, _dp(rho), _dp(ux), _dp(uy), _dp(uz)))
+        return rho, ux, uy, uz
+
+    def reduce_ref(self, subs=0):
+        out = np.empty(12)
+        _check(lib().lbm_reduce(self.h, REDUCE_REF, subs, _dp(out)))
+        return out
+
+    def reduce_sums(self):
+        out = np.empty(6)
+        _check(lib().lbm_reduce(self.h, REDUCE_SUMS, 0, _dp(out)))
+        return out
+
+    def last_flux(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().lbm_last_flux(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def profile(self, on=True):
+        _check(lib().lbm_profile_enable(self.h, int(bool(on))))
+
+    def profile_read(self):
+        ms = (ctypes.c_double * 5)()
+        cnt = (ctypes.c_int * 5)()
+        _check(lib().lbm_profile_read(self.h, ms, cnt))
+        return {k: (ms[q], cnt[q]) for q, k in enumerate(PROFILE_CLASSES)}
+
+
+def discs2d(ni, nj, seed, rmin, rmax, phi, walls=True):
+    m = np.empty(ni * nj, np.uint8)
+    ph = ctypes.c_double()
+    _check(lib().lbm_geom_discs2d(ni, nj, seed, rmin, rmax, phi, int(walls),
+                                  m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(ph)))
+    return m.reshape(nj, ni), ph.value
+
+
+def spheres3d(ni, nj, nk, seed, rmin, rmax, phi, zbuf=4):
+    m = np.empty(ni * nj * nk, np.uint8)
+    ph = ctypes.c_double()
+    _check(lib().lbm_geom_spheres3d(ni, nj, nk, seed, rmin, rmax, phi, zbuf,
+                                    m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(ph)))
+    return m.reshape(nk, nj, ni), ph.value

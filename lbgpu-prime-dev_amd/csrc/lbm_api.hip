@@ -1,0 +1,791 @@
+// C ABI of the stream-collide library: handle, lattice set-up, time loop
+// pieces, transfers, reductions and timing.  See include/lbm_mi355x.h.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "lbm_aux.hpp"
+#include "lbm_device.hpp"
+#include "lbm_internal.h"
+#include "lbm_mi355x.h"
+
+using namespace lbm;
+
+static thread_local std::string g_err;
+
+extern "C" int lbm_fail(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return 1;
+}
+
+extern "C" const char *lbm_last_error(void) { return g_err.c_str(); }
+
+#define HIP_OK(call)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess) return lbm_fail("%s:%d %s: %s", __FILE__, __LINE__, #call,      \
+                                          hipGetErrorString(e_));                         \
+  } while (0)
+
+typedef _Float16 half_t;
+
+namespace {
+
+struct ProfPair {
+  hipEvent_t a, b;
+  int cls;
+};
+
+}  // namespace
+
+struct lbm_handle {
+  lbm_params p;
+  std::vector<double> series[4];  // n[0], n[1], s[0], s[1]
+  std::vector<double> cmap_host;
+  int D, Q, U, L;  // dims, directions, unbound directions, last-axis length
+  long N, E;
+  int nf = 0;
+  long S = 0, nbS = 0;
+  int nblk8 = 0;
+  size_t st_bytes = 8;
+  void *F = nullptr;
+  int *nbr = nullptr;
+  double *mv = nullptr;
+  int *node = nullptr;  // compact -> dense
+  std::vector<int> fid; // dense -> compact or -1
+  std::vector<int> node_host;
+  int *face_lo = nullptr, *face_hi = nullptr, *face_flux = nullptr;
+  int n_lo = 0, n_hi = 0, n_flux = 0;
+  int *plane_lo = nullptr, *plane_hi = nullptr, *tgt_lo = nullptr, *tgt_hi = nullptr;
+  void *halo_pos = nullptr, *halo_neg = nullptr;
+  double *cmap = nullptr;
+  double *red = nullptr;      // reduction partials
+  double *flux_dev = nullptr; // [2]
+  double flux = 0., fmass = 0.;
+  int flux_valid = 0;
+  int time = 0;
+  hipStream_t stream = nullptr;
+  int profile = 0;
+  std::vector<ProfPair> pending;
+  std::vector<hipEvent_t> pool;
+  double prof_ms[5] = {0, 0, 0, 0, 0};
+  int prof_n[5] = {0, 0, 0, 0, 0};
+};
+
+namespace {
+
+constexpr int kRedBlocks = 512;
+
+hipEvent_t ev_get(lbm_handle *h) {
+  if (!h->pool.empty()) {
+    hipEvent_t e = h->pool.back();
+    h->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct Prof {
+  lbm_handle *h;
+  int cls;
+  hipEvent_t a = nullptr;
+  Prof(lbm_handle *h_, int c) : h(h_), cls(c) {
+    if (h->profile) {
+      a = ev_get(h);
+      if (a) (void)hipEventRecord(a, h->stream);
+    }
+  }
+  ~Prof() {
+    if (h->profile && a) {
+      hipEvent_t b = ev_get(h);
+      if (b) {
+        (void)hipEventRecord(b, h->stream);
+        h->pending.push_back({a, b, cls});
+      }
+    }
+  }
+};
+
+template <typename ST, typename CT>
+StepArgs<ST, CT> make_args(const lbm_handle *h) {
+  StepArgs<ST, CT> A;
+  const lbm_params &p = h->p;
+  A.F = (ST *)h->F;
+  A.S = h->S;
+  A.nbr = h->nbr;
+  A.nbS = h->nbS;
+  A.nf = h->nf;
+  A.nblk8 = h->nblk8;
+  A.mv = h->mv;
+  A.write_mv = 0;
+  A.solute_on = 0;
+  A.rhoA = (CT)p.rho_A[0];
+  A.tau = (CT)p.tau[0];
+  A.inv_tau = (CT)(1. / p.tau[0]);
+  for (int d = 0; d < 3; d++) A.g[d] = (CT)p.gaccel[d];
+  A.rhoA_s = (CT)p.rho_A[1];
+  A.inv_tau_s = (CT)(p.nsubs == 2 ? 1. / p.tau[1] : 1.);
+  A.sink_fac = (CT)(1.0 - p.sink * p.tau[1]);
+  for (int s = 0; s < 2; s++) {
+    double sc = p.store_scale[s];
+    if (h->p.storage == LBM_STORE_F16) {
+      if (sc <= 0) sc = s == 0 ? 16384.0 : 16.0;
+    } else {
+      sc = 1.0;
+    }
+    A.scale[s] = (CT)sc;
+    A.inv_scale[s] = (CT)(1.0 / sc);
+  }
+  A.c0 = (CT)(h->p.storage == LBM_STORE_F64 ? 0.0 : p.store_shift_c);
+  return A;
+}
+
+// Calls fn.template operator()<Q, NS, ST, CT>() for the handle's kind.
+template <typename Fn>
+int dispatch(const lbm_handle *h, Fn &&fn) {
+  const int st = h->p.storage, ct = h->p.compute;
+#define LBM_DISPATCH_T(QQ, NN)                                                        \
+  if (st == LBM_STORE_F64 && ct == LBM_COMPUTE_F64) return fn.template operator()<QQ, NN, double, double>(); \
+  if (st == LBM_STORE_F64 && ct == LBM_COMPUTE_F32) return fn.template operator()<QQ, NN, double, float>();  \
+  if (st == LBM_STORE_F32 && ct == LBM_COMPUTE_F64) return fn.template operator()<QQ, NN, float, double>();  \
+  if (st == LBM_STORE_F32) return fn.template operator()<QQ, NN, float, float>();                            \
+  if (st == LBM_STORE_F16) return fn.template operator()<QQ, NN, half_t, float>();
+  if (h->Q == 9 && h->p.nsubs == 1) { LBM_DISPATCH_T(9, 1) }
+  if (h->Q == 9 && h->p.nsubs == 2) { LBM_DISPATCH_T(9, 2) }
+  if (h->Q == 19 && h->p.nsubs == 1) { LBM_DISPATCH_T(19, 1) }
+  if (h->Q == 19 && h->p.nsubs == 2) { LBM_DISPATCH_T(19, 2) }
+#undef LBM_DISPATCH_T
+  return lbm_fail("unsupported storage/compute combination");
+}
+
+inline unsigned blocks_for(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+void free_lattice(lbm_handle *h) {
+  void *ptrs[] = {h->F, h->nbr, h->mv, h->node, h->face_lo, h->face_hi, h->face_flux, h->plane_lo,
+                  h->plane_hi, h->tgt_lo, h->tgt_hi, h->halo_pos, h->halo_neg};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  h->F = nullptr;
+  h->nbr = nullptr;
+  h->mv = nullptr;
+  h->node = nullptr;
+  h->face_lo = h->face_hi = h->face_flux = nullptr;
+  h->plane_lo = h->plane_hi = h->tgt_lo = h->tgt_hi = nullptr;
+  h->halo_pos = h->halo_neg = nullptr;
+  h->nf = 0;
+}
+
+int need_lattice(const lbm_handle *h) {
+  if (!h) return lbm_fail("null handle");
+  if (!h->F) return lbm_fail("lbm_set_solids has not been called");
+  return 0;
+}
+
+// rho selection of bcs_1 / bcs_2 with time series and ramp (src/bcs.c:287-319)
+double bc_rho(const lbm_handle *h, int s, int which, int t) {
+  const lbm_params &p = h->p;
+  double rho, ramp;
+  const std::vector<double> &sn = h->series[s], &ss = h->series[2 + s];
+  if (which == 0 && p.pressure_n_in[s] == 2 && !sn.empty())
+    rho = p.pfadd + p.pfmul * sn[t % sn.size()];
+  else if (which == 2 && p.pressure_s_in[s] == 2 && !ss.empty())
+    rho = p.pfadd + p.pfmul * ss[t % ss.size()];
+  else if (which == 0 || which == 2)
+    rho = p.rho_in;
+  else
+    rho = p.rho_out;
+  if (p.bc_ramp_start >= 0 && p.bc_ramp_stop >= 0) {
+    if (t < p.bc_ramp_start) {
+      rho = p.rho_A[s];
+    } else if (t < p.bc_ramp_stop) {
+      ramp = ((double)(t - p.bc_ramp_start)) / ((double)(p.bc_ramp_stop - p.bc_ramp_start));
+      rho *= ramp;
+      rho += (1. - ramp) * p.rho_A[s];
+    }
+  }
+  return rho;
+}
+
+int upload_ints(int **dst, const std::vector<int> &v) {
+  if (*dst) (void)hipFree(*dst);
+  *dst = nullptr;
+  size_t n = v.empty() ? 1 : v.size();
+  HIP_OK(hipMalloc(dst, n * sizeof(int)));
+  if (!v.empty()) HIP_OK(hipMemcpy(*dst, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int launch_half(lbm_handle *h, int odd, int t, int write_mv) {
+  return dispatch(h, [&]<int Q, int NS, typename ST, typename CT>() -> int {
+    StepArgs<ST, CT> A = make_args<ST, CT>(h);
+    A.write_mv = write_mv;
+    A.solute_on = NS == 2 && t >= h->p.sigma_t_on && t <= h->p.sigma_t_off;
+    Prof pr(h, odd ? 0 : 1);
+    if (odd)
+      hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, true>), dim3(8 * h->nblk8), dim3(kBlock), 0, h->stream, A);
+    else
+      hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, false>), dim3(8 * h->nblk8), dim3(kBlock), 0, h->stream, A);
+    HIP_OK(hipGetLastError());
+    return 0;
+  });
+}
+
+int launch_bcs(lbm_handle *h, int phase, int t) {
+  const lbm_params &p = h->p;
+  return dispatch(h, [&]<int Q, int NS, typename ST, typename CT>() -> int {
+    BcArgs<ST, CT> B;
+    B.A = make_args<ST, CT>(h);
+    B.phase = phase;
+    Prof pr(h, 2);
+    if constexpr (Q == 9) {
+      for (int s = 0; s < p.nsubs; s++) {
+        struct { int on, north, which; } pr4[4] = {{p.pressure_n_in[s], 1, 0}, {p.pressure_n_out[s], 1, 1},
+                                                   {p.pressure_s_in[s], 0, 2}, {p.pressure_s_out[s], 0, 3}};
+        for (auto &c : pr4) {
+          if (!c.on) continue;
+          B.list = c.north ? h->face_hi : h->face_lo;
+          B.n = c.north ? h->n_hi : h->n_lo;
+          if (!B.n) continue;
+          CT rb = (CT)bc_rho(h, s, c.which, t);
+          if (s == 0)
+            hipLaunchKernelGGL((k_bc2d_pressure<ST, CT, 0>), dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B, c.north, rb);
+          else
+            hipLaunchKernelGGL((k_bc2d_pressure<ST, CT, 1>), dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B, c.north, rb);
+        }
+        if (s == 1 && NS == 2) {
+          if (p.zeroconcgrad_n && h->n_hi) {
+            B.list = h->face_hi;
+            B.n = h->n_hi;
+            hipLaunchKernelGGL((k_bc2d_zerograd<ST, CT>), dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B, 1);
+          }
+          if (p.zeroconcgrad_s && h->n_lo) {
+            B.list = h->face_lo;
+            B.n = h->n_lo;
+            hipLaunchKernelGGL((k_bc2d_zerograd<ST, CT>), dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B, 0);
+          }
+        }
+      }
+    } else {
+      int sol = NS == 2 && p.zface_solute;
+      if (p.zface_pressure || sol) {
+        for (int top = 0; top < 2; top++) {
+          ZfaceArgs Z;
+          Z.top = top;
+          Z.flow = p.zface_pressure;
+          Z.solute = sol;
+          Z.rho_b = top ? p.rho_top : p.rho_bot;
+          Z.cmap = h->cmap;
+          Z.c_b = p.c_bot;
+          Z.node = h->node;
+          B.list = top ? h->face_hi : h->face_lo;
+          B.n = top ? h->n_hi : h->n_lo;
+          if (!B.n) continue;
+          hipLaunchKernelGGL((k_bc_zface<ST, CT, NS>), dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B, Z);
+        }
+      }
+    }
+    HIP_OK(hipGetLastError());
+    return 0;
+  });
+}
+
+HaloMaps halo_maps(const lbm_handle *h) {
+  HaloMaps H;
+  H.plane_lo = h->plane_lo;
+  H.plane_hi = h->plane_hi;
+  H.tgt_lo = h->tgt_lo;
+  H.tgt_hi = h->tgt_hi;
+  H.E = (int)h->E;
+  H.pairs = (h->Q - h->U) / 2;
+  H.U = h->U;
+  H.ghost = h->nf;
+  return H;
+}
+
+int halo_pack_unpack(lbm_handle *h, int phase, int pack, void *pos, void *neg) {
+  if (!h->p.halo_mode) return lbm_fail("halo planes need halo_mode = 1");
+  if (phase != 1 && phase != 2) return lbm_fail("phase must be 1 or 2");
+  return dispatch(h, [&]<int Q, int NS, typename ST, typename CT>() -> int {
+    StepArgs<ST, CT> A = make_args<ST, CT>(h);
+    HaloMaps H = halo_maps(h);
+    long total = (long)h->p.nsubs * H.pairs * H.E;
+    Prof pr(h, 3);
+    if (pack)
+      hipLaunchKernelGGL((k_halo_pack<Q, ST, CT>), dim3(blocks_for(total)), dim3(256), 0, h->stream, A, H,
+                         h->p.nsubs, phase, (ST *)pos, (ST *)neg);
+    else
+      hipLaunchKernelGGL((k_halo_unpack<Q, ST, CT>), dim3(blocks_for(total)), dim3(256), 0, h->stream, A, H,
+                         h->p.nsubs, phase, (const ST *)pos, (const ST *)neg);
+    HIP_OK(hipGetLastError());
+    return 0;
+  });
+}
+
+int launch_flux(lbm_handle *h, int phase) {
+  return dispatch(h, [&]<int Q, int NS, typename ST, typename CT>() -> int {
+    if constexpr (NS == 2) {
+      BcArgs<ST, CT> B;
+      B.A = make_args<ST, CT>(h);
+      B.phase = phase;
+      B.list = h->face_flux;
+      B.n = h->n_flux;
+      Prof pr(h, 4);
+      hipLaunchKernelGGL((k_plane_flux<Q, ST, CT>), dim3(1), dim3(256), 0, h->stream, B, h->flux_dev);
+      HIP_OK(hipGetLastError());
+    }
+    return 0;
+  });
+}
+
+int do_step(lbm_handle *h, int write_mv) {
+  int t = h->time + 1;
+  int odd = t & 1;
+  int phase = odd ? 1 : 2;
+  if (h->p.halo_mode) {
+    if (halo_pack_unpack(h, phase, 1, h->halo_pos, h->halo_neg)) return 1;
+    if (halo_pack_unpack(h, phase, 0, h->halo_pos, h->halo_neg)) return 1;
+  }
+  if (launch_bcs(h, phase, t)) return 1;
+  int sol_on = h->p.nsubs == 2 && t >= h->p.sigma_t_on && t <= h->p.sigma_t_off;
+  if (h->n_flux && sol_on) {
+    if (launch_flux(h, phase)) return 1;
+    h->flux_valid = 1;
+  }
+  if (launch_half(h, odd, t, write_mv)) return 1;
+  h->time = t;
+  return 0;
+}
+
+int sync(lbm_handle *h) {
+  HIP_OK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int lbm_create(const lbm_params *p, lbm_handle **out) {
+  if (!p || !out) return lbm_fail("lbm_create: null argument");
+  *out = nullptr;
+  if (p->dims != 2 && p->dims != 3) return lbm_fail("dims must be 2 or 3");
+  if (p->ni <= 0 || p->nj <= 0 || (p->dims == 3 && p->nk <= 0)) return lbm_fail("bad lattice size");
+  if (p->nsubs != 1 && p->nsubs != 2) return lbm_fail("nsubs must be 1 or 2");
+  for (int s = 0; s < p->nsubs; s++)
+    if (!(p->tau[s] > 0.5)) return lbm_fail("tau[%d] = %g must exceed 1/2", s, p->tau[s]);
+  if (p->storage < 0 || p->storage > 2 || p->compute < 0 || p->compute > 1)
+    return lbm_fail("bad storage/compute");
+  if (p->storage == LBM_STORE_F16 && p->compute == LBM_COMPUTE_F64)
+    return lbm_fail("FP16 storage runs the FP32 collision");
+  long N = (long)p->ni * p->nj * (p->dims == 3 ? p->nk : 1);
+  if (N >= (1L << 31) - (1L << 24)) return lbm_fail("lattice too large for 32-bit node indices");
+  lbm_handle *h = new lbm_handle();
+  h->p = *p;
+  if (p->dims == 2) h->p.nk = 1;
+  h->D = p->dims;
+  h->Q = p->dims == 2 ? 9 : 19;
+  h->U = p->dims == 2 ? 3 : 9;
+  h->L = p->dims == 2 ? p->nj : p->nk;
+  h->N = N;
+  h->E = p->dims == 2 ? p->ni : (long)p->ni * p->nj;
+  h->st_bytes = p->storage == LBM_STORE_F64 ? 8 : (p->storage == LBM_STORE_F32 ? 4 : 2);
+  for (int s = 0; s < 2; s++) {
+    if (p->pressure_n_series[s] && p->pressure_n_series_len[s] > 0)
+      h->series[s].assign(p->pressure_n_series[s], p->pressure_n_series[s] + p->pressure_n_series_len[s]);
+    if (p->pressure_s_series[s] && p->pressure_s_series_len[s] > 0)
+      h->series[2 + s].assign(p->pressure_s_series[s], p->pressure_s_series[s] + p->pressure_s_series_len[s]);
+    h->p.pressure_n_series[s] = nullptr;
+    h->p.pressure_s_series[s] = nullptr;
+  }
+  h->p.c_bot_map = nullptr;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return lbm_fail("hipStreamCreate failed");
+  }
+  if (p->c_bot_map && p->dims == 3) {
+    size_t E = (size_t)h->E;
+    if (hipMalloc(&h->cmap, E * sizeof(double)) != hipSuccess ||
+        hipMemcpy(h->cmap, p->c_bot_map, E * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+      lbm_destroy(h);
+      return lbm_fail("inlet map upload failed");
+    }
+  }
+  if (hipMalloc(&h->red, sizeof(double) * 12 * kRedBlocks) != hipSuccess ||
+      hipMalloc(&h->flux_dev, sizeof(double) * 2) != hipSuccess) {
+    lbm_destroy(h);
+    return lbm_fail("hipMalloc failed");
+  }
+  *out = h;
+  return 0;
+}
+
+extern "C" int lbm_destroy(lbm_handle *h) {
+  if (!h) return 0;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  free_lattice(h);
+  if (h->cmap) (void)hipFree(h->cmap);
+  if (h->red) (void)hipFree(h->red);
+  if (h->flux_dev) (void)hipFree(h->flux_dev);
+  for (auto &pp : h->pending) {
+    (void)hipEventDestroy(pp.a);
+    (void)hipEventDestroy(pp.b);
+  }
+  for (auto e : h->pool) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+extern "C" int lbm_set_solids(lbm_handle *h, const uint8_t *mask) {
+  if (!h || !mask) return lbm_fail("lbm_set_solids: null argument");
+  HIP_OK(hipStreamSynchronize(h->stream));
+  free_lattice(h);
+  const long N = h->N, E = h->E;
+  const int L = h->L;
+  h->fid.assign(N, -1);
+  h->node_host.clear();
+  for (long n = 0; n < N; n++)
+    if (mask[n] != 1) {
+      h->fid[n] = (int)h->node_host.size();
+      h->node_host.push_back((int)n);
+    }
+  const int nf = (int)h->node_host.size();
+  h->nf = nf;
+  if (nf == 0) return lbm_fail("lattice has no fluid node");
+  long rows = nf + (h->p.halo_mode ? 2 * E : 0);
+  h->S = (rows + 63) / 64 * 64;
+  h->nbS = ((long)nf + 63) / 64 * 64;
+  long nblk = blocks_for(nf, kBlock);
+  h->nblk8 = (int)((nblk + 7) / 8);
+  const int Q = h->Q, ns = h->p.nsubs;
+
+  HIP_OK(hipMalloc(&h->F, (size_t)ns * Q * h->S * h->st_bytes));
+  HIP_OK(hipMemsetAsync(h->F, 0, (size_t)ns * Q * h->S * h->st_bytes, h->stream));
+  HIP_OK(hipMalloc(&h->nbr, (size_t)(Q - 1) * h->nbS * sizeof(int)));
+  HIP_OK(hipMalloc(&h->mv, (size_t)ns * (1 + h->D) * nf * sizeof(double)));
+  HIP_OK(hipMemsetAsync(h->mv, 0, (size_t)ns * (1 + h->D) * nf * sizeof(double), h->stream));
+  if (upload_ints(&h->node, h->node_host)) return 1;
+  int *fid_dev = nullptr;
+  HIP_OK(hipMalloc(&fid_dev, (size_t)N * sizeof(int)));
+  HIP_OK(hipMemcpy(fid_dev, h->fid.data(), (size_t)N * sizeof(int), hipMemcpyHostToDevice));
+  if (Q == 9)
+    hipLaunchKernelGGL(k_build_nbr<9>, dim3(blocks_for(nf)), dim3(256), 0, h->stream, fid_dev, h->node, nf,
+                       h->p.ni, h->p.nj, h->p.nk, h->p.halo_mode, nf, h->nbr, h->nbS);
+  else
+    hipLaunchKernelGGL(k_build_nbr<19>, dim3(blocks_for(nf)), dim3(256), 0, h->stream, fid_dev, h->node, nf,
+                       h->p.ni, h->p.nj, h->p.nk, h->p.halo_mode, nf, h->nbr, h->nbS);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(h->stream));
+  HIP_OK(hipFree(fid_dev));
+
+  // faces along the last axis and the flux plane
+  std::vector<int> lo, hi, fl;
+  for (long m = 0; m < E; m++) {
+    if (h->fid[m] >= 0) lo.push_back(h->fid[m]);
+    if (h->fid[(L - 1) * E + m] >= 0) hi.push_back(h->fid[(L - 1) * E + m]);
+    if (h->p.flux_plane >= 0 && h->p.flux_plane < L && h->fid[(long)h->p.flux_plane * E + m] >= 0)
+      fl.push_back(h->fid[(long)h->p.flux_plane * E + m]);
+  }
+  h->n_lo = (int)lo.size();
+  h->n_hi = (int)hi.size();
+  h->n_flux = h->p.nsubs == 2 ? (int)fl.size() : 0;
+  if (upload_ints(&h->face_lo, lo) || upload_ints(&h->face_hi, hi) || upload_ints(&h->face_flux, fl)) return 1;
+
+  if (h->p.halo_mode) {
+    const int ni = h->p.ni, nj = h->D == 3 ? h->p.nj : 1, pairs = (Q - h->U) / 2;
+    std::vector<int> pl(E), ph(E), tl((size_t)pairs * E), th((size_t)pairs * E);
+    for (long m = 0; m < E; m++) {
+      pl[m] = h->fid[m];
+      ph[m] = h->fid[(L - 1) * E + m];
+    }
+    for (int q = 0; q < pairs; q++) {
+      int a = h->U + 2 * q;
+      for (long m = 0; m < E; m++) {
+        int i = (int)(m % ni), j = (int)(m / ni);
+        int iu = (i + CX(a) + ni) % ni, ju = h->D == 3 ? (j + CY(a) + nj) % nj : 0;
+        int id = (i - CX(a) + ni) % ni, jd = h->D == 3 ? (j - CY(a) + nj) % nj : 0;
+        bool up_fluid = h->fid[iu + (long)ni * ju] >= 0;               // pusher above the last plane
+        bool dn_fluid = h->fid[(L - 1) * E + id + (long)ni * jd] >= 0; // pusher below the first plane
+        th[(size_t)q * E + m] = up_fluid ? ph[m] : -1;
+        tl[(size_t)q * E + m] = dn_fluid ? pl[m] : -1;
+      }
+    }
+    if (upload_ints(&h->plane_lo, pl) || upload_ints(&h->plane_hi, ph) || upload_ints(&h->tgt_lo, tl) ||
+        upload_ints(&h->tgt_hi, th))
+      return 1;
+    size_t hb = (size_t)h->p.nsubs * pairs * E * h->st_bytes;
+    HIP_OK(hipMalloc(&h->halo_pos, hb));
+    HIP_OK(hipMalloc(&h->halo_neg, hb));
+  }
+  HIP_OK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+static int transfer_f(lbm_handle *h, int subs, double *f1d, int up) {
+  if (need_lattice(h)) return 1;
+  if (subs < 0 || subs >= h->p.nsubs) return lbm_fail("bad subs %d", subs);
+  if (!f1d) return lbm_fail("null population array");
+  const int Q = h->Q, nf = h->nf;
+  std::vector<double> st((size_t)Q * nf);
+  double *dstage = nullptr;
+  HIP_OK(hipMalloc(&dstage, st.size() * sizeof(double)));
+  if (up) {
+    for (int a = 0; a < Q; a++)
+      for (int x = 0; x < nf; x++) st[(size_t)a * nf + x] = f1d[(long)a * h->N + h->node_host[x]];
+    HIP_OK(hipMemcpy(dstage, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  int rc = dispatch(h, [&]<int QQ, int NS, typename ST, typename CT>() -> int {
+    StepArgs<ST, CT> A = make_args<ST, CT>(h);
+    dim3 g(blocks_for(nf)), b(256);
+    if (up) {
+      if (subs == 0) hipLaunchKernelGGL((k_encode<QQ, ST, CT, 0>), g, b, 0, h->stream, A, dstage);
+      else hipLaunchKernelGGL((k_encode<QQ, ST, CT, 1>), g, b, 0, h->stream, A, dstage);
+    } else {
+      if (subs == 0) hipLaunchKernelGGL((k_decode<QQ, ST, CT, 0>), g, b, 0, h->stream, A, dstage);
+      else hipLaunchKernelGGL((k_decode<QQ, ST, CT, 1>), g, b, 0, h->stream, A, dstage);
+    }
+    HIP_OK(hipGetLastError());
+    return 0;
+  });
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) rc = lbm_fail("transfer: %s", hipGetErrorString(e));
+  }
+  if (!rc && !up) {
+    hipError_t e = hipMemcpy(st.data(), dstage, st.size() * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = lbm_fail("transfer: %s", hipGetErrorString(e));
+    else {
+      memset(f1d, 0, sizeof(double) * Q * h->N);
+      for (int a = 0; a < Q; a++)
+        for (int x = 0; x < nf; x++) f1d[(long)a * h->N + h->node_host[x]] = st[(size_t)a * nf + x];
+    }
+  }
+  (void)hipFree(dstage);
+  return rc;
+}
+
+extern "C" int lbm_upload_f(lbm_handle *h, int subs, const double *f1d) {
+  return transfer_f(h, subs, const_cast<double *>(f1d), 1);
+}
+
+extern "C" int lbm_download_f(lbm_handle *h, int subs, double *f1d) { return transfer_f(h, subs, f1d, 0); }
+
+extern "C" int lbm_init_rest(lbm_handle *h, const double *c0) {
+  if (need_lattice(h)) return 1;
+  double *dc = nullptr;
+  if (c0 && h->p.nsubs == 2) {
+    std::vector<double> cc(h->nf);
+    for (int x = 0; x < h->nf; x++) cc[x] = c0[h->node_host[x]];
+    HIP_OK(hipMalloc(&dc, cc.size() * sizeof(double)));
+    HIP_OK(hipMemcpy(dc, cc.data(), cc.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  int rc = dispatch(h, [&]<int Q, int NS, typename ST, typename CT>() -> int {
+    StepArgs<ST, CT> A = make_args<ST, CT>(h);
+    hipLaunchKernelGGL((k_init_rest<Q, NS, ST, CT>), dim3(blocks_for(h->nf)), dim3(256), 0, h->stream, A, dc);
+    HIP_OK(hipGetLastError());
+    return 0;
+  });
+  if (!rc) rc = sync(h);
+  if (dc) (void)hipFree(dc);
+  return rc;
+}
+
+extern "C" int lbm_set_time(lbm_handle *h, int t) {
+  if (!h) return lbm_fail("null handle");
+  h->time = t;
+  return 0;
+}
+
+extern "C" int lbm_get_time(const lbm_handle *h, int *t) {
+  if (!h || !t) return lbm_fail("null argument");
+  *t = h->time;
+  return 0;
+}
+
+extern "C" int lbm_step(lbm_handle *h, int write_macro) {
+  if (need_lattice(h)) return 1;
+  if (do_step(h, write_macro)) return 1;
+  return sync(h);
+}
+
+extern "C" int lbm_run(lbm_handle *h, int nsteps, int write_macro_last) {
+  if (need_lattice(h)) return 1;
+  for (int q = 0; q < nsteps; q++)
+    if (do_step(h, write_macro_last && q == nsteps - 1)) return 1;
+  return sync(h);
+}
+
+extern "C" int lbm_step_odd(lbm_handle *h, int t, int write_macro) {
+  if (need_lattice(h)) return 1;
+  if (launch_half(h, 1, t, write_macro)) return 1;
+  return sync(h);
+}
+
+extern "C" int lbm_step_even(lbm_handle *h, int t, int write_macro) {
+  if (need_lattice(h)) return 1;
+  if (launch_half(h, 0, t, write_macro)) return 1;
+  return sync(h);
+}
+
+extern "C" int lbm_apply_bcs(lbm_handle *h, int phase, int t) {
+  if (need_lattice(h)) return 1;
+  if (phase != 1 && phase != 2) return lbm_fail("phase must be 1 or 2");
+  if (launch_bcs(h, phase, t)) return 1;
+  return sync(h);
+}
+
+extern "C" int lbm_halo_bytes(const lbm_handle *h, long *bytes) {
+  if (!h || !bytes) return lbm_fail("null argument");
+  *bytes = (long)h->p.nsubs * ((h->Q - h->U) / 2) * h->E * (long)h->st_bytes;
+  return 0;
+}
+
+extern "C" int lbm_halo_pack(lbm_handle *h, int phase, void *pos, void *neg) {
+  if (need_lattice(h)) return 1;
+  if (!pos || !neg) return lbm_fail("null halo buffer");
+  if (halo_pack_unpack(h, phase, 1, pos, neg)) return 1;
+  return sync(h);
+}
+
+extern "C" int lbm_halo_unpack(lbm_handle *h, int phase, const void *pos, const void *neg) {
+  if (need_lattice(h)) return 1;
+  if (!pos || !neg) return lbm_fail("null halo buffer");
+  if (halo_pack_unpack(h, phase, 0, const_cast<void *>(pos), const_cast<void *>(neg))) return 1;
+  return sync(h);
+}
+
+extern "C" int lbm_halo_periodic(lbm_handle *h, int phase) {
+  if (need_lattice(h)) return 1;
+  if (halo_pack_unpack(h, phase, 1, h->halo_pos, h->halo_neg)) return 1;
+  if (halo_pack_unpack(h, phase, 0, h->halo_pos, h->halo_neg)) return 1;
+  return sync(h);
+}
+
+extern "C" int lbm_download_macro(lbm_handle *h, int subs, double *rho, double *ux, double *uy, double *uz) {
+  if (need_lattice(h)) return 1;
+  if (subs < 0 || subs >= h->p.nsubs) return lbm_fail("bad subs %d", subs);
+  HIP_OK(hipStreamSynchronize(h->stream));
+  const int nf = h->nf, D = h->D;
+  std::vector<double> buf((size_t)(1 + D) * nf);
+  HIP_OK(hipMemcpy(buf.data(), h->mv + (long)subs * (1 + D) * nf, buf.size() * sizeof(double),
+                   hipMemcpyDeviceToHost));
+  double *dst[4] = {rho, ux, uy, uz};
+  for (int c = 0; c < 4; c++) {
+    if (!dst[c]) continue;
+    memset(dst[c], 0, sizeof(double) * h->N);
+    if (c > D) continue;
+    for (int x = 0; x < nf; x++) dst[c][h->node_host[x]] = buf[(size_t)c * nf + x];
+  }
+  return 0;
+}
+
+extern "C" int lbm_reduce(lbm_handle *h, int kind, int subs, double *out) {
+  if (need_lattice(h)) return 1;
+  if (!out) return lbm_fail("null output");
+  if (kind == LBM_REDUCE_SUMS) subs = 0;
+  if (subs < 0 || subs >= h->p.nsubs) return lbm_fail("bad subs %d", subs);
+  const int D = h->D, nf = h->nf;
+  double res[2][12];
+  int nsub_red = kind == LBM_REDUCE_SUMS ? h->p.nsubs : 1;
+  for (int r = 0; r < nsub_red; r++) {
+    int s = kind == LBM_REDUCE_SUMS ? r : subs;
+    const double *mv = h->mv + (long)s * (1 + D) * nf;
+    double off = (s == 0) ? h->p.rho_A[0] : 0.0;
+    if (D == 2)
+      hipLaunchKernelGGL(k_reduce_mv<2>, dim3(kRedBlocks), dim3(256), 0, h->stream, mv, nf, off, h->red);
+    else
+      hipLaunchKernelGGL(k_reduce_mv<3>, dim3(kRedBlocks), dim3(256), 0, h->stream, mv, nf, off, h->red);
+    HIP_OK(hipGetLastError());
+    std::vector<double> part((size_t)12 * kRedBlocks);
+    HIP_OK(hipMemcpyAsync(part.data(), h->red, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    double v[12];
+    for (int q = 0; q < 12; q++) v[q] = part[q];
+    for (int b = 1; b < kRedBlocks; b++)
+      for (int q = 0; q < 12; q++) {
+        double x = part[(size_t)b * 12 + q];
+        v[q] = q < 4 ? v[q] + x : (q < 8 ? (x > v[q] ? x : v[q]) : (x < v[q] ? x : v[q]));
+      }
+    for (int q = 0; q < 12; q++) res[r][q] = v[q];
+  }
+  if (kind == LBM_REDUCE_REF) {
+    double *v = res[0];
+    double off = subs == 0 ? h->p.rho_A[0] : 0.0;
+    out[0] = v[4];
+    out[1] = v[8];
+    out[2] = off + v[0] / nf;
+    for (int c = 0; c < 3; c++) {
+      out[3 + c] = c < D ? v[5 + c] : 0.0;
+      out[6 + c] = c < D ? v[9 + c] : 0.0;
+      out[9 + c] = c < D ? v[1 + c] / nf : 0.0;
+    }
+    return 0;
+  }
+  if (kind == LBM_REDUCE_SUMS) {
+    out[0] = res[0][0];
+    out[1] = res[0][1];
+    out[2] = res[0][2];
+    out[3] = D == 3 ? res[0][3] : 0.0;
+    out[4] = (double)nf;
+    out[5] = h->p.nsubs == 2 ? res[1][0] : 0.0;
+    return 0;
+  }
+  return lbm_fail("unknown reduction %d", kind);
+}
+
+extern "C" int lbm_last_flux(lbm_handle *h, double *flux, double *mass) {
+  if (need_lattice(h)) return 1;
+  if (!flux || !mass) return lbm_fail("null output");
+  if (!h->flux_valid) return lbm_fail("no plane flux recorded (flux_plane < 0 or no solute)");
+  double v[2];
+  HIP_OK(hipMemcpyAsync(v, h->flux_dev, sizeof v, hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  *flux = v[0];
+  *mass = v[1];
+  return 0;
+}
+
+extern "C" int lbm_info(const lbm_handle *h, long *n_nodes, long *n_fluid, long *row_stride) {
+  if (!h) return lbm_fail("null handle");
+  if (n_nodes) *n_nodes = h->N;
+  if (n_fluid) *n_fluid = h->nf;
+  if (row_stride) *row_stride = h->S;
+  return 0;
+}
+
+extern "C" int lbm_profile_enable(lbm_handle *h, int on) {
+  if (!h) return lbm_fail("null handle");
+  h->profile = on;
+  return 0;
+}
+
+extern "C" int lbm_profile_read(lbm_handle *h, double *ms, int *count) {
+  if (!h) return lbm_fail("null handle");
+  HIP_OK(hipStreamSynchronize(h->stream));
+  for (auto &pp : h->pending) {
+    float t = 0.f;
+    HIP_OK(hipEventElapsedTime(&t, pp.a, pp.b));
+    h->prof_ms[pp.cls] += t;
+    h->prof_n[pp.cls] += 1;
+    h->pool.push_back(pp.a);
+    h->pool.push_back(pp.b);
+  }
+  h->pending.clear();
+  for (int c = 0; c < 5; c++) {
+    if (ms) ms[c] = h->prof_ms[c];
+    if (count) count[c] = h->prof_n[c];
+    h->prof_ms[c] = 0;
+    h->prof_n[c] = 0;
+  }
+  return 0;
+}

@@ -1,0 +1,237 @@
+// Auxiliary device kernels: compaction tables, population upload/download,
+// halo planes, plane flux and FP64 reductions.
+#pragma once
+
+#include "lbm_device.hpp"
+
+namespace lbm {
+
+// Neighbour table: nbr[a-1][x] = compact index of x - c_a (periodic wrap in
+// every axis, or through ghost entries across the last axis in halo mode),
+// -1 when that node is solid.  Ghost entries: [ghost, ghost+E) stand for the
+// plane below the first one, [ghost+E, ghost+2E) for the plane above the
+// last one; their solidity is that of the periodic image.
+template <int Q>
+__global__ void k_build_nbr(const int *fid, const int *node, int nf, int ni, int nj, int nk, int halo,
+                            int ghost, int *nbr, long nbS) {
+  constexpr int D = Q == 9 ? 2 : 3;
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nf) return;
+  const long n = node[idx];
+  const long E = D == 2 ? (long)ni : (long)ni * nj;
+  const int L = D == 2 ? nj : nk;  // last axis
+  int i = (int)(n % ni);
+  int j = (int)((n / ni) % nj);
+  int k = D == 2 ? 0 : (int)(n / E);
+#pragma unroll
+  for (int a = 1; a < Q; a++) {
+    int ii = i - CX(a), jj = j - CY(a), kk = k - CZ(a);
+    ii = ii < 0 ? ii + ni : (ii >= ni ? ii - ni : ii);
+    int last = D == 2 ? jj : kk;
+    if (D == 3) jj = jj < 0 ? jj + nj : (jj >= nj ? jj - nj : jj);
+    int r;
+    if (last >= 0 && last < L) {
+      long m = D == 2 ? (long)ii + (long)ni * last : (long)ii + (long)ni * jj + E * last;
+      r = fid[m];
+    } else if (!halo) {
+      last = last < 0 ? last + L : last - L;
+      long m = D == 2 ? (long)ii + (long)ni * last : (long)ii + (long)ni * jj + E * last;
+      r = fid[m];
+    } else {
+      long p = D == 2 ? (long)ii : (long)ii + (long)ni * jj;
+      if (last < 0)
+        r = fid[p + E * (L - 1)] >= 0 ? ghost + (int)p : -1;
+      else
+        r = fid[p] >= 0 ? ghost + (int)(E + p) : -1;
+    }
+    nbr[(long)(a - 1) * nbS + idx] = r;
+  }
+}
+
+// staging [Q][nf] doubles <-> storage rows of one subs
+template <int Q, typename ST, typename CT, int SUBS>
+__global__ void k_encode(StepArgs<ST, CT> A, const double *stage) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= A.nf) return;
+  ST *Fs = A.F + (long)SUBS * Q * A.S;
+#pragma unroll
+  for (int a = 0; a < Q; a++)
+    Fs[(long)a * A.S + idx] =
+        Codec<ST, CT>::enc((CT)stage[(long)a * A.nf + idx], A.scale[SUBS], shift_of<Q, ST, CT, SUBS>(A, a));
+}
+
+template <int Q, typename ST, typename CT, int SUBS>
+__global__ void k_decode(StepArgs<ST, CT> A, double *stage) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= A.nf) return;
+  const ST *Fs = A.F + (long)SUBS * Q * A.S;
+#pragma unroll
+  for (int a = 0; a < Q; a++)
+    stage[(long)a * A.nf + idx] =
+        (double)Codec<ST, CT>::dec(Fs[(long)a * A.S + idx], A.inv_scale[SUBS], shift_of<Q, ST, CT, SUBS>(A, a));
+}
+
+// rest state: f~ = w rho (3 c.u + 4.5 (c.u)^2 - 1.5 u^2) = 0 at u = 0;
+// solute g = w c0 (src/latman.c:1786-1893)
+template <int Q, int NS, typename ST, typename CT>
+__global__ void k_init_rest(StepArgs<ST, CT> A, const double *c0) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= A.nf) return;
+#pragma unroll
+  for (int a = 0; a < Q; a++) {
+    double v = W<Q>(a) * 1.0 * (3. * 0. + 4.5 * 0. * 0. - 1.5 * 0.);
+    A.F[(long)a * A.S + idx] = Codec<ST, CT>::enc((CT)v, A.scale[0], (CT)0);
+  }
+  if (NS == 2) {
+    double c = c0 ? c0[idx] : 0.;
+    ST *G = A.F + (long)Q * A.S;
+#pragma unroll
+    for (int a = 0; a < Q; a++) {
+      double v = W<Q>(a) * c * (1. + 3. * 0. + 4.5 * 0. * 0. - 1.5 * 0.);
+      G[(long)a * A.S + idx] = Codec<ST, CT>::enc((CT)v, A.scale[1], shift_of<Q, ST, CT, 1>(A, a));
+    }
+  }
+}
+
+// ---- halo planes: k_bound_DtH_{1,2} / k_bound_HtD_{1,2} ----------------
+// message layout [subs][pair q][E], pair q = directions (U+2q, U+2q+1)
+struct HaloMaps {
+  const int *plane_lo;  // [E] compact index of the first-plane node or -1
+  const int *plane_hi;  // [E] last plane
+  const int *tgt_lo;    // [pairs][E] phase-2 target in the first plane or -1
+  const int *tgt_hi;    // [pairs][E] phase-2 target in the last plane or -1
+  int E, pairs, U, ghost;
+};
+
+template <int Q, typename ST, typename CT>
+__global__ void k_halo_pack(StepArgs<ST, CT> A, HaloMaps H, int nsubs, int phase, ST *pos, ST *neg) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long total = (long)nsubs * H.pairs * H.E;
+  if (t >= total) return;
+  int m = (int)(t % H.E);
+  int q = (int)((t / H.E) % H.pairs);
+  int s = (int)(t / ((long)H.E * H.pairs));
+  int a = H.U + 2 * q;
+  const ST *Fs = A.F + (long)s * Q * A.S;
+  if (phase == 1) {
+    int hi = H.plane_hi[m], lo = H.plane_lo[m];
+    pos[t] = hi >= 0 ? Fs[(long)a * A.S + hi] : (ST)0;
+    neg[t] = lo >= 0 ? Fs[(long)(a + 1) * A.S + lo] : (ST)0;
+  } else {
+    neg[t] = Fs[(long)a * A.S + H.ghost + m];
+    pos[t] = Fs[(long)(a + 1) * A.S + H.ghost + H.E + m];
+  }
+}
+
+template <int Q, typename ST, typename CT>
+__global__ void k_halo_unpack(StepArgs<ST, CT> A, HaloMaps H, int nsubs, int phase, const ST *pos,
+                              const ST *neg) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long total = (long)nsubs * H.pairs * H.E;
+  if (t >= total) return;
+  int m = (int)(t % H.E);
+  int q = (int)((t / H.E) % H.pairs);
+  int s = (int)(t / ((long)H.E * H.pairs));
+  int a = H.U + 2 * q;
+  ST *Fs = A.F + (long)s * Q * A.S;
+  if (phase == 1) {
+    Fs[(long)a * A.S + H.ghost + m] = pos[t];
+    Fs[(long)(a + 1) * A.S + H.ghost + H.E + m] = neg[t];
+  } else {
+    int hi = H.tgt_hi[(long)q * H.E + m], lo = H.tgt_lo[(long)q * H.E + m];
+    if (hi >= 0) Fs[(long)a * A.S + hi] = neg[t];
+    if (lo >= 0) Fs[(long)(a + 1) * A.S + lo] = pos[t];
+  }
+}
+
+// ---- FP64 reductions -----------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// kind: 0 sum, 1 max, 2 min.  Reduces K values over the block (256
+// threads = 4 waves) in a fixed order.
+template <int K>
+__device__ void block_reduce(double (&v)[K], const int (&kind)[K], double *out) {
+  __shared__ double lds[4][K];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    double x = kind[q] == 0 ? wave_sum(v[q]) : (kind[q] == 1 ? wave_max(v[q]) : wave_min(v[q]));
+    if (lane == 0) lds[wv][q] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    int q = threadIdx.x;
+    double x = lds[0][q];
+    for (int w = 1; w < 4; w++)
+      x = kind[q] == 0 ? x + lds[w][q] : (kind[q] == 1 ? fmax(x, lds[w][q]) : fmin(x, lds[w][q]));
+    out[q] = x;
+  }
+}
+
+// per block: [0] sum(rho - rho_A) (or sum c), [1..3] sum u, [4] max|rho|,
+// [5..7] max|u|, [8] min|rho|, [9..11] min|u|
+template <int D>
+__global__ __launch_bounds__(256) void k_reduce_mv(const double *mv, int nf, double rho_off, double *partial) {
+  double v[12];
+  constexpr int kinds_init[12] = {0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
+  int kind[12];
+#pragma unroll
+  for (int q = 0; q < 12; q++) {
+    kind[q] = kinds_init[q];
+    v[q] = q < 8 ? 0.0 : __builtin_huge_val();
+  }
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < nf; idx += (long)gridDim.x * blockDim.x) {
+    double r = mv[idx];
+    v[0] += r - rho_off;
+    v[4] = fmax(v[4], fabs(r));
+    v[8] = fmin(v[8], fabs(r));
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      double u = d < D ? mv[(long)(d + 1) * nf + idx] : 0.0;
+      v[1 + d] += u;
+      v[5 + d] = fmax(v[5 + d], fabs(u));
+      v[9 + d] = fmin(v[9 + d], fabs(u));
+    }
+  }
+  block_reduce<12>(v, kind, partial + (long)blockIdx.x * 12);
+}
+
+// solute moments of one plane: sum_a c_a,last g_a and sum_a g_a on the
+// populations the coming half-step collides (one block, fixed order)
+template <int Q, typename ST, typename CT>
+__global__ __launch_bounds__(256) void k_plane_flux(BcArgs<ST, CT> B, double *out) {
+  double v[2] = {0.0, 0.0};
+  const int kind[2] = {0, 0};
+  for (int t = threadIdx.x; t < B.n; t += blockDim.x) {
+    const int idx = B.list[t];
+    int nb[Q];
+    CT f[Q];
+    bc_gather<Q, ST, CT, 1>(B, idx, nb, f);
+    double jl = 0.0, m = 0.0;
+#pragma unroll
+    for (int a = 0; a < Q; a++) {
+      jl += (double)(Q == 9 ? CY(a) : CZ(a)) * (double)f[a];
+      m += (double)f[a];
+    }
+    v[0] += jl;
+    v[1] += m;
+  }
+  block_reduce<2>(v, kind, out);
+}
+
+}  // namespace lbm

@@ -1,0 +1,384 @@
+// Device code of the stream-collide library (gfx950, wave64).
+//
+// Storage: populations of fluid nodes only, SoA rows [subs][slot a][S]
+// indexed by the compact fluid index (dense order, i fastest).  Slot
+// meaning follows the reference's in-place AA scheme (Bailey et al., ICPP
+// 2009; src/k_stream_collide_stream.c, src/k_collide.c): before an odd
+// half-step slot a at x holds the post-collision f_a(x); the odd half-step
+// pulls slot a from x - c_a, collides and pushes into slot opp(a) at
+// x + c_a; the even half-step reads the own node's opposite slots and writes
+// its own slots in order.  Halfway bounce-back is folded into a neighbour
+// table: nbr[a][x] is the compact index of x - c_a, or -1 when that node is
+// solid (then the own opposite slot is read / the own slot is written).
+//
+// All arithmetic is written in the reference's order; the library is built
+// with -ffp-contract=off so the FP64 path reproduces the FP64 oracle bit for
+// bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lbm {
+
+constexpr int kBlock = 256;
+
+// D3Q19 velocity set in the reference's order (src/lbgpu_prime.h:78-84):
+// opposite directions adjacent, D2Q9 first.
+__host__ __device__ constexpr int CX(int a) {
+  constexpr int t[19] = {0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0};
+  return t[a];
+}
+__host__ __device__ constexpr int CY(int a) {
+  constexpr int t[19] = {0, 0, 0, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 0, 0, 1, -1, -1, 1};
+  return t[a];
+}
+__host__ __device__ constexpr int CZ(int a) {
+  constexpr int t[19] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1};
+  return t[a];
+}
+__host__ __device__ constexpr int OPP(int a) { return a == 0 ? 0 : ((a & 1) ? a + 1 : a - 1); }
+
+// weights of src/latman.c:161-192
+template <int Q>
+__host__ __device__ constexpr double W(int a) {
+  if (Q == 9) return a == 0 ? 4. / 9. : (a < 5 ? 1. / 9. : 1. / 36.);
+  return a == 0 ? 1. / 3. : ((a < 5 || a == 9 || a == 10) ? 1. / 18. : 1. / 36.);
+}
+
+// population coding in storage
+template <typename ST, typename CT>
+struct Codec {
+  static __device__ __forceinline__ CT dec(ST v, CT inv_scale, CT shift) {
+    return (CT)v * inv_scale + shift;
+  }
+  static __device__ __forceinline__ ST enc(CT v, CT scale, CT shift) { return (ST)((v - shift) * scale); }
+};
+template <>
+struct Codec<double, double> {
+  static __device__ __forceinline__ double dec(double v, double, double) { return v; }
+  static __device__ __forceinline__ double enc(double v, double, double) { return v; }
+};
+template <>
+struct Codec<double, float> {
+  static __device__ __forceinline__ float dec(double v, float, float) { return (float)v; }
+  static __device__ __forceinline__ double enc(float v, float, float) { return (double)v; }
+};
+
+template <typename ST, typename CT>
+struct StepArgs {
+  ST *F;          // [nsubs][Q][S]
+  long S;         // slots per direction row
+  const int *nbr; // [Q-1][nbS]
+  long nbS;
+  int nf;         // fluid nodes
+  int nblk8;      // launched blocks / 8
+  double *mv;     // [nsubs][1+D][nf]
+  int write_mv;
+  int solute_on;
+  CT rhoA, tau, inv_tau, g[3];
+  CT rhoA_s, inv_tau_s, sink_fac;
+  CT scale[2], inv_scale[2], c0;
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs; map them so that
+// each XCD sweeps one contiguous range of fluid nodes (its own L2 keeps the
+// partial lines that neighbouring waves share).
+__device__ __forceinline__ int xcd_block(int nblk8) {
+  int b = blockIdx.x;
+  return (b & 7) * nblk8 + (b >> 3);
+}
+
+template <int Q, typename ST, typename CT, int SUBS>
+__device__ __forceinline__ CT shift_of(const StepArgs<ST, CT> &A, int a) {
+  return SUBS == 1 ? (CT)W<Q>(a) * A.c0 : (CT)0;
+}
+
+// Gather the Q populations one half-step collides.
+template <int Q, typename ST, typename CT, int SUBS, bool ODD>
+__device__ __forceinline__ void gather(const StepArgs<ST, CT> &A, int idx, const int (&nb)[Q],
+                                       CT (&f)[Q]) {
+  const ST *Fs = A.F + (long)SUBS * Q * A.S;
+#pragma unroll
+  for (int a = 0; a < Q; a++) {
+    long off;
+    if (a == 0)
+      off = idx;
+    else if (ODD)
+      off = nb[a] >= 0 ? (long)a * A.S + nb[a] : (long)OPP(a) * A.S + idx;
+    else
+      off = (long)OPP(a) * A.S + idx;
+    f[a] = Codec<ST, CT>::dec(Fs[off], A.inv_scale[SUBS], shift_of<Q, ST, CT, SUBS>(A, a));
+  }
+}
+
+template <int Q, typename ST, typename CT, int SUBS, bool ODD>
+__device__ __forceinline__ void scatter(const StepArgs<ST, CT> &A, int idx, const int (&nb)[Q],
+                                        const CT (&f)[Q]) {
+  ST *Fs = A.F + (long)SUBS * Q * A.S;
+#pragma unroll
+  for (int a = 0; a < Q; a++) {
+    long off;
+    if (a == 0)
+      off = idx;
+    else if (ODD)
+      off = nb[OPP(a)] >= 0 ? (long)OPP(a) * A.S + nb[OPP(a)] : (long)a * A.S + idx;
+    else
+      off = (long)a * A.S + idx;
+    Fs[off] = Codec<ST, CT>::enc(f[a], A.scale[SUBS], shift_of<Q, ST, CT, SUBS>(A, a));
+  }
+}
+
+// Moments, force, macroscopic write and BGK relaxation of one subs, in the
+// order of src/k_stream_collide_stream.c:80-390 and
+// src/lattice.c:2850-2944.  SUBS 0: He-Luo incompressible flow on shifted
+// populations; SUBS 1: Inamuro solute advected by the flow's u_eq.
+template <int Q, typename ST, typename CT, int SUBS, bool ODD>
+__device__ __forceinline__ void collide(const StepArgs<ST, CT> &A, int idx, CT (&f)[Q], CT (&u)[3]) {
+  constexpr int D = Q == 9 ? 2 : 3;
+  CT rho = f[0];
+#pragma unroll
+  for (int a = 1; a < Q; a++) rho += f[a];
+  if (SUBS == 0) {
+    u[0] = (CT)0;
+    u[1] = (CT)0;
+    u[2] = (CT)0;
+#pragma unroll
+    for (int a = 0; a < Q; a++) {
+      u[0] += ((CT)CX(a)) * f[a];
+      u[1] += ((CT)CY(a)) * f[a];
+      if (D == 3) u[2] += ((CT)CZ(a)) * f[a];
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++) u[d] /= A.rhoA;
+#pragma unroll
+    for (int d = 0; d < D; d++) u[d] += A.tau * A.g[d];
+  } else {
+    if (!ODD) rho *= A.sink_fac;
+    rho *= A.sink_fac;
+  }
+  if (A.write_mv) {
+    double *mv = A.mv + (long)SUBS * (1 + D) * A.nf;
+    if (SUBS == 0)
+      mv[idx] = (double)(rho + A.rhoA);
+    else
+      mv[idx] = (double)rho;
+#pragma unroll
+    for (int d = 0; d < D; d++) mv[(long)(d + 1) * A.nf + idx] = (double)u[d];
+  }
+  CT usq = (CT)0;
+  if (SUBS == 0) {
+    usq = u[0] * u[0] + u[1] * u[1];
+    if (D == 3) usq += u[2] * u[2];
+  }
+  const CT it = SUBS == 0 ? A.inv_tau : A.inv_tau_s;
+#pragma unroll
+  for (int a = 0; a < Q; a++) {
+    CT fa = f[a] * ((CT)1 - it);
+    CT vdotu = ((CT)CX(a)) * u[0] + ((CT)CY(a)) * u[1];
+    if (D == 3) vdotu += ((CT)CZ(a)) * u[2];
+    const CT w = (CT)W<Q>(a);
+    if (SUBS == 1) {
+      fa += w * rho * ((CT)1 + (CT)3 * vdotu) * it;
+    } else {
+      fa += w * rho * it;
+      fa += w * A.rhoA * ((CT)3 * vdotu + (CT)4.5 * vdotu * vdotu - (CT)1.5 * usq) * it;
+    }
+    f[a] = fa;
+  }
+}
+
+template <int Q, int NS, typename ST, typename CT, bool ODD>
+__global__ __launch_bounds__(kBlock) void k_half_step(StepArgs<ST, CT> A) {
+  const int idx = xcd_block(A.nblk8) * kBlock + (int)threadIdx.x;
+  if (idx >= A.nf) return;
+  int nb[Q];
+  nb[0] = idx;
+  if (ODD) {
+#pragma unroll
+    for (int a = 1; a < Q; a++) nb[a] = A.nbr[(long)(a - 1) * A.nbS + idx];
+  }
+  CT f[Q], u[3];
+  gather<Q, ST, CT, 0, ODD>(A, idx, nb, f);
+  collide<Q, ST, CT, 0, ODD>(A, idx, f, u);
+  scatter<Q, ST, CT, 0, ODD>(A, idx, nb, f);
+  if (NS == 2 && A.solute_on) {
+    gather<Q, ST, CT, 1, ODD>(A, idx, nb, f);
+    collide<Q, ST, CT, 1, ODD>(A, idx, f, u);
+    scatter<Q, ST, CT, 1, ODD>(A, idx, nb, f);
+  }
+}
+
+// ---- boundary conditions ------------------------------------------------
+
+template <typename ST, typename CT>
+struct BcArgs {
+  StepArgs<ST, CT> A;
+  const int *list;  // compact indices of the face's fluid nodes
+  int n;
+  int phase;        // 1: before the odd half-step, 2: before the even one
+};
+
+template <int Q, typename ST, typename CT, int SUBS>
+__device__ __forceinline__ void bc_gather(const BcArgs<ST, CT> &B, int idx, int (&nb)[Q], CT (&f)[Q]) {
+  nb[0] = idx;
+  if (B.phase == 1) {
+#pragma unroll
+    for (int a = 1; a < Q; a++) nb[a] = B.A.nbr[(long)(a - 1) * B.A.nbS + idx];
+    gather<Q, ST, CT, SUBS, true>(B.A, idx, nb, f);
+  } else {
+    gather<Q, ST, CT, SUBS, false>(B.A, idx, nb, f);
+  }
+}
+
+// store the value the coming half-step will collide as f_a:
+//   phase 1 -> the slot the pull reads (slot a at x - c_a); when x - c_a is
+//   solid the reference writes the own slot a (set_f1d_d, da = 0,
+//   src/lattice.c:2835-2844) unless own_opp, which writes the own slot
+//   opp(a) that the pull then reads;  phase 2 -> own slot opp(a).
+template <int Q, typename ST, typename CT, int SUBS>
+__device__ __forceinline__ void bc_store(const BcArgs<ST, CT> &B, int idx, const int (&nb)[Q], int a,
+                                         CT v, bool own_opp) {
+  ST *Fs = B.A.F + (long)SUBS * Q * B.A.S;
+  long off;
+  if (B.phase == 1)
+    off = nb[a] >= 0 ? (long)a * B.A.S + nb[a] : (long)(own_opp ? OPP(a) : a) * B.A.S + idx;
+  else
+    off = (long)OPP(a) * B.A.S + idx;
+  Fs[off] = Codec<ST, CT>::enc(v, B.A.scale[SUBS], shift_of<Q, ST, CT, SUBS>(B.A, a));
+}
+
+// 2-D Zou-He pressure on the N (j = nj-1) or S (j = 0) row:
+// k_sysbound_pressure_{n,s}_{1,2}, src/system_boundary_kernels.c:4-451
+template <typename ST, typename CT, int SUBS>
+__global__ void k_bc2d_pressure(BcArgs<ST, CT> B, int north, CT rho_b) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B.n) return;
+  const int idx = B.list[t];
+  int nb[9];
+  CT f[9];
+  bc_gather<9, ST, CT, SUBS>(B, idx, nb, f);
+  enum { C_ = 0, E_, W_, N_, S_, NE_, SW_, NW_, SE_ };
+  const CT rhoA = SUBS == 0 ? B.A.rhoA : B.A.rhoA_s;
+  if (north) {
+    CT x = f[C_] + f[E_] + f[W_] + f[N_] + f[N_] + f[NE_] + f[NE_] + f[NW_] + f[NW_];
+    x -= rho_b;
+    x += rhoA;
+    CT vS = f[N_] - ((CT)(2. / 3.)) * x;
+    CT vSW = f[NE_] + ((CT)(1. / 2.)) * (f[E_] - f[W_]) - ((CT)(1. / 6.)) * x;
+    CT vSE = f[NW_] + ((CT)(1. / 2.)) * (f[W_] - f[E_]) - ((CT)(1. / 6.)) * x;
+    bc_store<9, ST, CT, SUBS>(B, idx, nb, S_, vS, false);
+    bc_store<9, ST, CT, SUBS>(B, idx, nb, SE_, vSE, false);
+    bc_store<9, ST, CT, SUBS>(B, idx, nb, SW_, vSW, false);
+  } else {
+    CT x = -f[C_] - f[E_] - f[W_] - f[S_] - f[S_] - f[SE_] - f[SE_] - f[SW_] - f[SW_];
+    x += rho_b;
+    x -= rhoA;
+    CT vN = f[S_] + ((CT)(2. / 3.)) * x;
+    CT vNE = f[SW_] + ((CT)(1. / 2.)) * (f[W_] - f[E_]) + ((CT)(1. / 6.)) * x;
+    CT vNW = f[SE_] + ((CT)(1. / 2.)) * (f[E_] - f[W_]) + ((CT)(1. / 6.)) * x;
+    bc_store<9, ST, CT, SUBS>(B, idx, nb, N_, vN, false);
+    bc_store<9, ST, CT, SUBS>(B, idx, nb, NW_, vNW, false);
+    bc_store<9, ST, CT, SUBS>(B, idx, nb, NE_, vNE, false);
+  }
+}
+
+// 2-D solute zero concentration gradient on the N or S row:
+// k_sysbound_zeroconcgrad_{n,s}_{1,2}, src/system_boundary_kernels.c:454-831
+template <typename ST, typename CT>
+__global__ void k_bc2d_zerograd(BcArgs<ST, CT> B, int north) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B.n) return;
+  const int idx = B.list[t];
+  int nb[9];
+  CT f[9];
+  bc_gather<9, ST, CT, 1>(B, idx, nb, f);
+  enum { C_ = 0, E_, W_, N_, S_, NE_, SW_, NW_, SE_ };
+  constexpr double w1 = W<9>(1), w2 = W<9>(5);
+  if (north) {
+    CT c = (f[N_] + f[NE_] + f[NW_]) / ((CT)(w1 + w2 + w2));
+    bc_store<9, ST, CT, 1>(B, idx, nb, S_, (CT)w1 * c, false);
+    bc_store<9, ST, CT, 1>(B, idx, nb, SE_, (CT)w2 * c, false);
+    bc_store<9, ST, CT, 1>(B, idx, nb, SW_, (CT)w2 * c, false);
+  } else {
+    CT c = (f[S_] + f[SW_] + f[SE_]) / ((CT)(w1 + w2 + w2));
+    bc_store<9, ST, CT, 1>(B, idx, nb, N_, (CT)w1 * c, false);
+    bc_store<9, ST, CT, 1>(B, idx, nb, NW_, (CT)w2 * c, false);
+    bc_store<9, ST, CT, 1>(B, idx, nb, NE_, (CT)w2 * c, false);
+  }
+}
+
+// 3-D z faces (no reference counterpart).  Flow: Zou-He pressure with zero
+// tangential velocity in the D3Q19 form of Hecht & Harting (J. Stat. Mech.
+// 2010, P01018) on shifted populations; solute: Dirichlet c at z=0, zero
+// gradient at z=nk-1.  Unknowns replace what streaming would deliver.
+struct ZfaceArgs {
+  int top;
+  int flow;
+  int solute;
+  double rho_b;       // prescribed density of this face
+  const double *cmap; // [nj][ni] inlet concentration or null
+  double c_b;
+  const int *node;    // compact -> dense index
+};
+
+template <typename ST, typename CT, int NS>
+__global__ void k_bc_zface(BcArgs<ST, CT> B, ZfaceArgs Z) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B.n) return;
+  const int idx = B.list[t];
+  constexpr int UP[5] = {9, 11, 13, 15, 17};
+  constexpr int DN[5] = {10, 12, 14, 16, 18};
+  int nb[19];
+  CT f[19];
+  if (Z.flow) {
+    bc_gather<19, ST, CT, 0>(B, idx, nb, f);
+    CT P = f[0];
+#pragma unroll
+    for (int a = 1; a < 9; a++) P += f[a];
+    CT M = (CT)0;
+#pragma unroll
+    for (int q = 0; q < 5; q++) M += f[Z.top ? UP[q] : DN[q]];
+    CT Nx = ((f[1] + f[5] + f[8]) - (f[2] + f[7] + f[6])) / (CT)2;
+    CT Ny = ((f[3] + f[5] + f[7]) - (f[4] + f[6] + f[8])) / (CT)2;
+    CT drb = (CT)Z.rho_b - B.A.rhoA;
+    if (!Z.top) {
+      CT J = drb - (P + (CT)2 * M);
+      bc_store<19, ST, CT, 0>(B, idx, nb, 9, f[10] + J / (CT)3, true);
+      bc_store<19, ST, CT, 0>(B, idx, nb, 11, f[12] + J / (CT)6 - Nx, true);
+      bc_store<19, ST, CT, 0>(B, idx, nb, 13, f[14] + J / (CT)6 + Nx, true);
+      bc_store<19, ST, CT, 0>(B, idx, nb, 15, f[16] + J / (CT)6 - Ny, true);
+      bc_store<19, ST, CT, 0>(B, idx, nb, 17, f[18] + J / (CT)6 + Ny, true);
+    } else {
+      CT J = (P + (CT)2 * M) - drb;
+      bc_store<19, ST, CT, 0>(B, idx, nb, 10, f[9] - J / (CT)3, true);
+      bc_store<19, ST, CT, 0>(B, idx, nb, 12, f[11] - J / (CT)6 + Nx, true);
+      bc_store<19, ST, CT, 0>(B, idx, nb, 14, f[13] - J / (CT)6 - Nx, true);
+      bc_store<19, ST, CT, 0>(B, idx, nb, 16, f[15] - J / (CT)6 + Ny, true);
+      bc_store<19, ST, CT, 0>(B, idx, nb, 18, f[17] - J / (CT)6 - Ny, true);
+    }
+  }
+  if (NS == 2 && Z.solute) {
+    bc_gather<19, ST, CT, 1>(B, idx, nb, f);
+    CT known = (CT)0, cs;
+    if (!Z.top) {
+      CT cb = (CT)Z.c_b;
+      if (Z.cmap) cb = (CT)Z.cmap[Z.node[idx]];  // z = 0: dense index = i + ni j
+#pragma unroll
+      for (int a = 0; a < 9; a++) known += f[a];
+#pragma unroll
+      for (int q = 0; q < 5; q++) known += f[DN[q]];
+      cs = (cb - known) / (CT)(1. / 6.);
+#pragma unroll
+      for (int q = 0; q < 5; q++) bc_store<19, ST, CT, 1>(B, idx, nb, UP[q], (CT)W<19>(UP[q]) * cs, true);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; q++) known += f[UP[q]];
+      cs = known / (CT)(1. / 6.);
+#pragma unroll
+      for (int q = 0; q < 5; q++) bc_store<19, ST, CT, 1>(B, idx, nb, DN[q], (CT)W<19>(DN[q]) * cs, true);
+    }
+  }
+}
+
+}  // namespace lbm

@@ -1,0 +1,245 @@
+"""HIP product path against the FP64 CPU oracle, through the C ABI.
+
+FP64 storage + FP64 collision is required to match the oracle to 1e-12
+(normwise, per field, over fluid nodes); the kernels keep the reference's
+operation order without fused multiply-adds, so the expected difference is
+zero and the tests also report whether the match is bitwise."""
+import sys
+
+import numpy as np
+import pytest
+
+import _pkg
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def relnorm(a, b):
+    d = np.max(np.abs(a - b)) if a.size else 0.0
+    s = np.max(np.abs(b)) if b.size else 0.0
+    return d / s if s > 0 else d
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def build_pair(mask, **kw):
+    L = _pkg.load()
+    o = Oracle(**kw)
+    o.set_solids(mask)
+    g = L.Lattice(mask=mask, **kw)
+    return o, g
+
+
+def random_state(rng, Q, N, nsubs):
+    f = rng.uniform(-1e-3, 1e-3, (Q, N))
+    gg = rng.uniform(0.0, 0.1, (Q, N)) if nsubs == 2 else None
+    return f, gg
+
+
+def load_both(o, g, f, gg):
+    o.set_f(0, f)
+    g.upload_f(0, f)
+    if gg is not None:
+        o.set_f(1, gg)
+        g.upload_f(1, gg)
+
+
+def compare(o, g, mask, nsubs, label, tol=TOL, compare_f=True):
+    fl = np.asarray(mask).reshape(-1) != 1
+    worst = 0.0
+    bitwise = True
+    for s in range(nsubs):
+        mo, mg = o.mv(s), g.mv(s)
+        for c in range(1 + o.dims):
+            r = relnorm(mg[c][fl], mo[c][fl])
+            worst = max(worst, r)
+            bitwise &= bool(np.array_equal(mg[c][fl], mo[c][fl]))
+        if compare_f:
+            fo, fg = o.get_f(s), g.download_f(s)
+            r = relnorm(fg[:, fl], fo[:, fl])
+            worst = max(worst, r)
+            bitwise &= bool(np.array_equal(fg[:, fl], fo[:, fl]))
+    say(f"{label}: max normwise diff {worst:.3e} bitwise={bitwise}")
+    assert worst <= tol, f"{label}: {worst}"
+    return worst, bitwise
+
+
+@pytest.mark.parametrize("dims,shape,nsubs", [(2, (12, 17), 1), (2, (9, 10), 2), (3, (7, 6, 9), 1),
+                                               (3, (8, 5, 6), 2)])
+@pytest.mark.parametrize("nsteps", [1, 2, 7])
+def test_random_state_fp64(dims, shape, nsubs, nsteps):
+    say(f"case dims={dims} shape={shape} nsubs={nsubs} nsteps={nsteps}")
+    rng = np.random.default_rng(1000 + dims * 10 + nsubs + nsteps)
+    mask = (rng.random(shape) < 0.3).astype(np.uint8)
+    kw = dict(dims=dims, ni=shape[-1], nj=shape[-2], nk=shape[0] if dims == 3 else 1, nsubs=nsubs,
+              tau=(0.8, 0.7), gaccel=(1e-4, -2e-4, 3e-4, 0, 0, 0))
+    o, g = build_pair(mask, **kw)
+    Q = 9 if dims == 2 else 19
+    f, gg = random_state(rng, Q, mask.size, nsubs)
+    load_both(o, g, f, gg)
+    o.run(nsteps)
+    g.run(nsteps)
+    compare(o, g, mask, nsubs, f"random {dims}D nsubs={nsubs} steps={nsteps}")
+
+
+def test_c1_d2q9_discs_full_length():
+    """configuration 1: 128x128 D2Q9, tau=1, walls at y=0,127, discs to
+    porosity 0.70, g_x = 1e-6, rest start, 10,000 steps"""
+    L = _pkg.load()
+    say("case C1 128x128 10000 steps")
+    mask, phi = L.discs2d(128, 128, 101, 4.0, 10.0, 0.70, walls=True)
+    kw = dict(dims=2, ni=128, nj=128, tau=(1.0, 1.0), gaccel=(1e-6, 0, 0, 0, 0, 0))
+    o, g = build_pair(mask, **kw)
+    o.init_rest()
+    g.init_rest()
+    o.run(10000)
+    g.run(10000)
+    compare(o, g, mask, 1, "C1 10000 steps")
+    ro, rg = o.reduce_ref(0), g.reduce_ref(0)
+    say("C1 compute.c set oracle", ro[:3], "gpu", rg[:3])
+    assert relnorm(rg, ro) <= 1e-12
+
+
+@pytest.mark.parametrize("phase_kw", [
+    dict(pressure_n_out=(1, 0), pressure_s_in=(1, 0), rho_in=1.002, rho_out=0.999),
+    dict(pressure_s_in=(2, 0), pressure_n_out=(1, 0), rho_out=1.0, bc_ramp_start=3, bc_ramp_stop=15,
+         pfadd=0.5, pfmul=0.5),
+])
+def test_2d_reference_bcs_with_solute(phase_kw):
+    say(f"case 2D BCs {sorted(phase_kw)}")
+    rng = np.random.default_rng(77)
+    ni, nj = 24, 20
+    mask = (rng.random((nj, ni)) < 0.15).astype(np.uint8)
+    mask[0] = 0
+    mask[-1] = 0
+    mask[1] = 0
+    mask[-2] = 0
+    kw = dict(dims=2, ni=ni, nj=nj, nsubs=2, tau=(0.9, 0.75), zeroconcgrad_n=1, zeroconcgrad_s=1,
+              pressure_s_series=[np.array([2.002, 2.006, 1.998, 2.0]), None], **phase_kw)
+    o, g = build_pair(mask, **kw)
+    f, gg = random_state(rng, 9, mask.size, 2)
+    load_both(o, g, f, gg)
+    for t in range(40):
+        o.step(True)
+        g.step(True)
+    compare(o, g, mask, 2, "2D Zou-He + zero gradient")
+
+
+@pytest.mark.parametrize("halo", [0, 1])
+def test_3d_zface_pressure_solute_flux(halo):
+    say(f"case 3D z-face BCs halo={halo}")
+    L = _pkg.load()
+    ni, nj, nk = 20, 18, 32
+    mask, _ = L.spheres3d(ni, nj, nk, 404, 3.0, 6.0, 0.55, zbuf=3)
+    rng = np.random.default_rng(5)
+    cmap = rng.uniform(0.8, 1.2, (nj, ni))
+    kw = dict(dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(0.8, 0.75), zface_pressure=1, rho_bot=1.001,
+              rho_top=1.0, zface_solute=1, c_bot_map=cmap, flux_plane=nk - 2, halo_mode=halo)
+    o, g = build_pair(mask, **kw)
+    o.init_rest()
+    g.init_rest()
+    for t in range(1, 61):
+        o.step(t % 10 == 0)
+        g.step(t % 10 == 0)
+        fo, fg = o.last_flux(), g.last_flux()
+        assert abs(fg[0] - fo[0]) <= 1e-12 * max(abs(fo[0]), 1e-300) + 1e-300, (t, fo, fg)
+        assert abs(fg[1] - fo[1]) <= 1e-12 * abs(fo[1]) + 1e-300
+    compare(o, g, mask, 2, f"3D z-face halo={halo}")
+    so, sg = o.reduce_sums(), g.reduce_sums()
+    assert relnorm(sg, so) <= 1e-12
+
+
+class DeviceBuffer:
+    """device memory from the HIP runtime the library already loaded"""
+
+    def __init__(self, nbytes):
+        import ctypes
+        self.ct = ctypes
+        self.hip = ctypes.CDLL("libamdhip64.so.7")
+        self.nbytes = nbytes
+        self.ptr = ctypes.c_void_p()
+        assert self.hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(nbytes)) == 0
+        assert self.hip.hipMemset(self.ptr, 0, ctypes.c_size_t(nbytes)) == 0
+        assert self.hip.hipDeviceSynchronize() == 0
+
+    def numpy(self, dtype=np.float64):
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype)
+        assert self.hip.hipMemcpy(out.ctypes.data_as(self.ct.c_void_p), self.ptr,
+                                  self.ct.c_size_t(self.nbytes), 2) == 0
+        return out
+
+    def __del__(self):
+        try:
+            self.hip.hipFree(self.ptr)
+        except Exception:
+            pass
+
+
+def test_halo_pack_buffers_match_oracle():
+    """lbm_halo_pack fills device buffers with the reference's message
+    layout [subs][pair][LX*LY] (fluid positions compared); unpacking the
+    buffers of a periodic self-exchange reproduces lbm_halo_periodic"""
+    say("case halo pack buffers")
+    ni, nj, nk = 10, 8, 12
+    rng = np.random.default_rng(12)
+    mask = (rng.random((nk, nj, ni)) < 0.3).astype(np.uint8)
+    kw = dict(dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(0.8, 0.7), halo_mode=1)
+    o, g = build_pair(mask, **kw)
+    f, gg = random_state(rng, 19, mask.size, 2)
+    load_both(o, g, f, gg)
+    nbytes = g.halo_bytes()
+    E = ni * nj
+    assert nbytes == 2 * 5 * E * 8
+    pos, neg = DeviceBuffer(nbytes), DeviceBuffer(nbytes)
+    first = mask[0].reshape(-1) != 1
+    last = mask[-1].reshape(-1) != 1
+    g.halo_pack(1, pos.ptr.value, neg.ptr.value)
+    po, no = o.halo_pack(1)
+    pg, ng = pos.numpy().reshape(2, 5, E), neg.numpy().reshape(2, 5, E)
+    po, no = po.reshape(2, 5, E), no.reshape(2, 5, E)
+    assert np.array_equal(pg[:, :, last], po[:, :, last])
+    assert np.array_equal(ng[:, :, first], no[:, :, first])
+    # explicit pack/unpack through the buffers == the built-in periodic exchange
+    g.halo_unpack(1, pos.ptr.value, neg.ptr.value)
+    g.bcs_1(1)
+    g.stream_collide_stream(1, True)
+    g.halo_pack(2, pos.ptr.value, neg.ptr.value)
+    g.halo_unpack(2, pos.ptr.value, neg.ptr.value)
+    g.bcs_2(2)
+    g.collide(2, True)
+    g.time = 2
+    o.run(2)
+    compare(o, g, mask, 2, "explicit halo buffers, 2 steps")
+
+
+@pytest.mark.parametrize("storage,compute,tol", [(0, 1, 1e-3), (1, 1, 1e-3), (1, 0, 1e-4), (2, 1, 1e-1)])
+def test_reduced_precision_error_vs_fp64(storage, compute, tol):
+    """storage FP64/FP32/FP16 with FP32 collision against the FP64 oracle
+    (bounds are stated error levels, not parity)"""
+    say(f"case precision storage={storage}")
+    L = _pkg.load()
+    ni, nj, nk = 24, 20, 40
+    mask, _ = L.spheres3d(ni, nj, nk, 303, 3.0, 6.0, 0.5, zbuf=3)
+    kw = dict(dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(0.8, 0.75), zface_pressure=1, rho_bot=1.001,
+              rho_top=1.0, zface_solute=1, c_bot=1.0)
+    o = Oracle(**kw)
+    o.set_solids(mask)
+    o.init_rest()
+    g = L.Lattice(mask=mask, storage=storage, compute=compute, store_shift_c=0.5 if storage == 2 else 0.0,
+                  **kw)
+    g.init_rest()
+    o.run(300)
+    g.run(300)
+    fl = mask.reshape(-1) != 1
+    mo, mg = o.mv(0), g.mv(0)
+    er = relnorm(mg[0][fl] - 1, mo[0][fl] - 1)
+    eu = max(relnorm(mg[c][fl], mo[c][fl]) for c in (1, 2, 3))
+    ec = relnorm(g.mv(1)[0][fl], o.mv(1)[0][fl])
+    say(f"storage={storage} compute={compute}: drho err {er:.2e} u err {eu:.2e} c err {ec:.2e}")
+    assert eu <= tol and ec <= tol

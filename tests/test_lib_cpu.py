@@ -1,0 +1,61 @@
+"""CPU-side checks of the product library: it loads without a GPU, exports
+every entry point the C header declares, and its seeded geometry generator
+is reproducible and meets its stated properties."""
+import hashlib
+import os
+import re
+
+import numpy as np
+import pytest
+
+import _pkg
+
+ROOT = _pkg.ROOT
+
+
+def header_symbols():
+    src = open(os.path.join(ROOT, "include", "lbm_mi355x.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(lbm_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_header_symbols():
+    L = _pkg.load()
+    lib = L.lib()
+    syms = header_symbols()
+    assert len(syms) >= 25
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
+def test_discs2d_properties_and_hash():
+    L = _pkg.load()
+    m, phi = L.discs2d(128, 128, 101, 4.0, 10.0, 0.70, walls=True)
+    assert phi <= 0.70 and phi > 0.66
+    assert m[0].all() and m[-1].all()
+    assert abs(1 - m.mean() - phi) < 1e-12
+    # discs never reach the wall rows: the rows next to the walls are not all solid
+    assert not m[1].all() and not m[-2].all()
+    h = hashlib.sha256(m.tobytes()).hexdigest()
+    assert h == "7d02560889c5969c741a1501ce1b9ef35b44f6d163b68275fa249b467d4c87a6"
+    m2, _ = L.discs2d(128, 128, 101, 4.0, 10.0, 0.70, walls=True)
+    assert np.array_equal(m, m2)
+
+
+def test_spheres3d_properties():
+    L = _pkg.load()
+    m, phi = L.spheres3d(48, 40, 64, 202, 6.0, 14.0, 0.40, zbuf=4)
+    assert phi <= 0.40 and phi > 0.35
+    assert m[:4].sum() == 0 and m[-4:].sum() == 0
+    sample = m[4:-4]
+    assert abs((1 - sample.mean()) - phi) < 1e-12
+    m2, _ = L.spheres3d(48, 40, 64, 202, 6.0, 14.0, 0.40, zbuf=4)
+    assert np.array_equal(m, m2)
+    m3, _ = L.spheres3d(48, 40, 64, 203, 6.0, 14.0, 0.40, zbuf=4)
+    assert not np.array_equal(m, m3)
+
+
+def test_geometry_rejects_bad_arguments():
+    L = _pkg.load()
+    with pytest.raises(L.LbmError):
+        L.discs2d(16, 16, 1, 9.0, 10.0, 0.7, walls=True)
