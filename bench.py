@@ -101,6 +101,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
     L = load_pkg()
+    copy_gbps = L.copy_bandwidth(2 << 30, 20)
+    log(f"device copy bandwidth {copy_gbps:.1f} GB/s")
     n = args.size
     t0 = time.perf_counter()
     mask, phi = L.spheres3d(n, n, n, 202, 6.0, 14.0, 0.40, zbuf=4)
@@ -151,6 +153,8 @@ def main():
             "odd_frac": round(B / (odd_ms * 1e-3) / HBM_PEAK, 4) if odd_ms else None,
             "even_frac": round(B / (even_ms * 1e-3) / HBM_PEAK, 4) if even_ms else None,
             "step_frac_wall": round(step_bw / HBM_PEAK, 4),
+            "copy_gbps": round(copy_gbps, 1),
+            "dominant_frac_of_copy": round(achieved / (copy_gbps * 1e9), 4),
             "mass_sum": sums[0],
         },
     }

@@ -88,7 +88,8 @@ int lbm_destroy(lbm_handle *h);
 /* solids H2D with halo copies (src/lbgpu_prime.cu:170-257).  mask: N
  * bytes, i fastest; a node is solid when its byte equals 1, as on the
  * reference GPU path (src/lattice.c:2291).  Builds the fluid-node
- * compaction and the neighbour table; populations become zero. */
+ * compaction and the per-row fluid bitmasks the kernels derive neighbours
+ * from; populations become zero. */
 int lbm_set_solids(lbm_handle *h, const uint8_t *mask);
 
 /* f H2D / D2H per subs (src/lbgpu_prime.cu:317-343, :874-900).  Solid
@@ -143,7 +144,8 @@ int lbm_reduce(lbm_handle *h, int kind, int subs, double *out);
 /* solute moments of plane flux_plane in the last step */
 int lbm_last_flux(lbm_handle *h, double *flux, double *mass);
 
-/* geometry numbers: N, fluid nodes, stored slots per direction row */
+/* geometry numbers: N, fluid nodes, population slots per direction
+ * (fluid nodes and halo ghost entries rounded up to 64-node tiles) */
 int lbm_info(const lbm_handle *h, long *n_nodes, long *n_fluid, long *row_stride);
 
 /* Kernel timing with HIP events on the library's stream.  While enabled
@@ -153,6 +155,11 @@ int lbm_info(const lbm_handle *h, long *n_nodes, long *n_fluid, long *row_stride
  * [3] halo pack/unpack, [4] plane flux, and resets them. */
 int lbm_profile_enable(lbm_handle *h, int on);
 int lbm_profile_read(lbm_handle *h, double *ms, int *count);
+
+/* Device-to-device copy of nbytes (reps times) on a stream of its own:
+ * the measured bandwidth in GB/s (bytes read + written per second) is the
+ * practical ceiling that the half-steps are compared with. */
+int lbm_copy_bandwidth(long nbytes, int reps, double *gbps);
 
 /* Seeded synthetic porous media (xoshiro256** seeded by splitmix64).
  * 2-D: non-overlapping discs, radius U[rmin,rmax], periodic in x, solid

@@ -91,6 +91,7 @@ def lib():
             "lbm_info": [vp, lp, lp, lp],
             "lbm_profile_enable": [vp, ip],
             "lbm_profile_read": [vp, dp, ctypes.POINTER(ctypes.c_int)],
+            "lbm_copy_bandwidth": [ctypes.c_long, ip, dp],
             "lbm_geom_discs2d": [ip, ip, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_double, ip, u8p, dp],
             "lbm_geom_spheres3d": [ip, ip, ip, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
@@ -298,6 +299,13 @@ class Lattice:
         cnt = (ctypes.c_int * 5)()
         _check(lib().lbm_profile_read(self.h, ms, cnt))
         return {k: (ms[q], cnt[q]) for q, k in enumerate(PROFILE_CLASSES)}
+
+
+def copy_bandwidth(nbytes=4 << 30, reps=20):
+    """measured device-to-device copy bandwidth in GB/s"""
+    g = ctypes.c_double()
+    _check(lib().lbm_copy_bandwidth(int(nbytes), int(reps), ctypes.byref(g)))
+    return g.value
 
 
 def discs2d(ni, nj, seed, rmin, rmax, phi, walls=True):

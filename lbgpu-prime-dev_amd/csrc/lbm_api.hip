@@ -55,11 +55,12 @@ struct lbm_handle {
   int D, Q, U, L;  // dims, directions, unbound directions, last-axis length
   long N, E;
   int nf = 0;
-  long S = 0, nbS = 0;
+  long tiles = 0;  // 64-node tiles incl. ghost entries
+  int W = 0;       // 64-node words per lattice row
   int nblk8 = 0;
   size_t st_bytes = 8;
   void *F = nullptr;
-  int *nbr = nullptr;
+  RowWord *rw = nullptr;
   double *mv = nullptr;
   int *node = nullptr;  // compact -> dense
   std::vector<int> fid; // dense -> compact or -1
@@ -123,9 +124,17 @@ StepArgs<ST, CT> make_args(const lbm_handle *h) {
   StepArgs<ST, CT> A;
   const lbm_params &p = h->p;
   A.F = (ST *)h->F;
-  A.S = h->S;
-  A.nbr = h->nbr;
-  A.nbS = h->nbS;
+  A.G.rw = h->rw;
+  A.G.node = h->node;
+  A.G.ni = p.ni;
+  A.G.nj = p.nj;
+  A.G.nk = h->D == 3 ? p.nk : 1;
+  A.G.W = h->W;
+  A.G.halo = p.halo_mode;
+  A.G.ghost = h->nf;
+  A.G.E = (int)h->E;
+  A.G.div_ni = make_fastdiv((uint32_t)p.ni);
+  A.G.div_nj = make_fastdiv((uint32_t)p.nj);
   A.nf = h->nf;
   A.nblk8 = h->nblk8;
   A.mv = h->mv;
@@ -173,12 +182,12 @@ int dispatch(const lbm_handle *h, Fn &&fn) {
 inline unsigned blocks_for(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
 void free_lattice(lbm_handle *h) {
-  void *ptrs[] = {h->F, h->nbr, h->mv, h->node, h->face_lo, h->face_hi, h->face_flux, h->plane_lo,
+  void *ptrs[] = {h->F, h->rw, h->mv, h->node, h->face_lo, h->face_hi, h->face_flux, h->plane_lo,
                   h->plane_hi, h->tgt_lo, h->tgt_hi, h->halo_pos, h->halo_neg};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   h->F = nullptr;
-  h->nbr = nullptr;
+  h->rw = nullptr;
   h->mv = nullptr;
   h->node = nullptr;
   h->face_lo = h->face_hi = h->face_flux = nullptr;
@@ -233,10 +242,16 @@ int launch_half(lbm_handle *h, int odd, int t, int write_mv) {
     A.write_mv = write_mv;
     A.solute_on = NS == 2 && t >= h->p.sigma_t_on && t <= h->p.sigma_t_off;
     Prof pr(h, odd ? 0 : 1);
-    if (odd)
-      hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, true>), dim3(8 * h->nblk8), dim3(kBlock), 0, h->stream, A);
+    const bool off32 = (size_t)h->tiles * NS * Q * kTile * sizeof(ST) <= 0xffffffffull;
+    dim3 grid(8 * h->nblk8), block(kBlock);
+    if (odd && off32)
+      hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, true, true>), grid, block, 0, h->stream, A);
+    else if (odd)
+      hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, true, false>), grid, block, 0, h->stream, A);
+    else if (off32)
+      hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, false, true>), grid, block, 0, h->stream, A);
     else
-      hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, false>), dim3(8 * h->nblk8), dim3(kBlock), 0, h->stream, A);
+      hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, false, false>), grid, block, 0, h->stream, A);
     HIP_OK(hipGetLastError());
     return 0;
   });
@@ -260,11 +275,13 @@ int launch_bcs(lbm_handle *h, int phase, int t) {
           if (!B.n) continue;
           CT rb = (CT)bc_rho(h, s, c.which, t);
           if (s == 0)
-            hipLaunchKernelGGL((k_bc2d_pressure<ST, CT, 0>), dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B, c.north, rb);
-          else
-            hipLaunchKernelGGL((k_bc2d_pressure<ST, CT, 1>), dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B, c.north, rb);
+            hipLaunchKernelGGL((k_bc2d_pressure<NS, ST, CT, 0>), dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B,
+                               c.north, rb);
+          else if constexpr (NS == 2)
+            hipLaunchKernelGGL((k_bc2d_pressure<NS, ST, CT, 1>), dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B,
+                               c.north, rb);
         }
-        if (s == 1 && NS == 2) {
+        if constexpr (NS == 2) if (s == 1) {
           if (p.zeroconcgrad_n && h->n_hi) {
             B.list = h->face_hi;
             B.n = h->n_hi;
@@ -288,11 +305,10 @@ int launch_bcs(lbm_handle *h, int phase, int t) {
           Z.rho_b = top ? p.rho_top : p.rho_bot;
           Z.cmap = h->cmap;
           Z.c_b = p.c_bot;
-          Z.node = h->node;
           B.list = top ? h->face_hi : h->face_lo;
           B.n = top ? h->n_hi : h->n_lo;
           if (!B.n) continue;
-          hipLaunchKernelGGL((k_bc_zface<ST, CT, NS>), dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B, Z);
+          hipLaunchKernelGGL((k_bc_zface<NS, ST, CT>), dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B, Z);
         }
       }
     }
@@ -323,10 +339,10 @@ int halo_pack_unpack(lbm_handle *h, int phase, int pack, void *pos, void *neg) {
     long total = (long)h->p.nsubs * H.pairs * H.E;
     Prof pr(h, 3);
     if (pack)
-      hipLaunchKernelGGL((k_halo_pack<Q, ST, CT>), dim3(blocks_for(total)), dim3(256), 0, h->stream, A, H,
+      hipLaunchKernelGGL((k_halo_pack<Q, NS, ST, CT>), dim3(blocks_for(total)), dim3(256), 0, h->stream, A, H,
                          h->p.nsubs, phase, (ST *)pos, (ST *)neg);
     else
-      hipLaunchKernelGGL((k_halo_unpack<Q, ST, CT>), dim3(blocks_for(total)), dim3(256), 0, h->stream, A, H,
+      hipLaunchKernelGGL((k_halo_unpack<Q, NS, ST, CT>), dim3(blocks_for(total)), dim3(256), 0, h->stream, A, H,
                          h->p.nsubs, phase, (const ST *)pos, (const ST *)neg);
     HIP_OK(hipGetLastError());
     return 0;
@@ -342,7 +358,7 @@ int launch_flux(lbm_handle *h, int phase) {
       B.list = h->face_flux;
       B.n = h->n_flux;
       Prof pr(h, 4);
-      hipLaunchKernelGGL((k_plane_flux<Q, ST, CT>), dim3(1), dim3(256), 0, h->stream, B, h->flux_dev);
+      hipLaunchKernelGGL((k_plane_flux<Q, NS, ST, CT>), dim3(1), dim3(256), 0, h->stream, B, h->flux_dev);
       HIP_OK(hipGetLastError());
     }
     return 0;
@@ -462,31 +478,28 @@ extern "C" int lbm_set_solids(lbm_handle *h, const uint8_t *mask) {
   const int nf = (int)h->node_host.size();
   h->nf = nf;
   if (nf == 0) return lbm_fail("lattice has no fluid node");
-  long rows = nf + (h->p.halo_mode ? 2 * E : 0);
-  h->S = (rows + 63) / 64 * 64;
-  h->nbS = ((long)nf + 63) / 64 * 64;
+  long slots = nf + (h->p.halo_mode ? 2 * E : 0);
+  h->tiles = (slots + kTile - 1) / kTile;
   long nblk = blocks_for(nf, kBlock);
   h->nblk8 = (int)((nblk + 7) / 8);
   const int Q = h->Q, ns = h->p.nsubs;
+  const size_t elems = (size_t)h->tiles * ns * Q * kTile;
+  if (elems >= (1ull << 32)) return lbm_fail("lattice too large for 32-bit population offsets");
 
-  HIP_OK(hipMalloc(&h->F, (size_t)ns * Q * h->S * h->st_bytes));
-  HIP_OK(hipMemsetAsync(h->F, 0, (size_t)ns * Q * h->S * h->st_bytes, h->stream));
-  HIP_OK(hipMalloc(&h->nbr, (size_t)(Q - 1) * h->nbS * sizeof(int)));
+  HIP_OK(hipMalloc(&h->F, elems * h->st_bytes));
+  HIP_OK(hipMemsetAsync(h->F, 0, elems * h->st_bytes, h->stream));
   HIP_OK(hipMalloc(&h->mv, (size_t)ns * (1 + h->D) * nf * sizeof(double)));
   HIP_OK(hipMemsetAsync(h->mv, 0, (size_t)ns * (1 + h->D) * nf * sizeof(double), h->stream));
   if (upload_ints(&h->node, h->node_host)) return 1;
-  int *fid_dev = nullptr;
-  HIP_OK(hipMalloc(&fid_dev, (size_t)N * sizeof(int)));
-  HIP_OK(hipMemcpy(fid_dev, h->fid.data(), (size_t)N * sizeof(int), hipMemcpyHostToDevice));
-  if (Q == 9)
-    hipLaunchKernelGGL(k_build_nbr<9>, dim3(blocks_for(nf)), dim3(256), 0, h->stream, fid_dev, h->node, nf,
-                       h->p.ni, h->p.nj, h->p.nk, h->p.halo_mode, nf, h->nbr, h->nbS);
-  else
-    hipLaunchKernelGGL(k_build_nbr<19>, dim3(blocks_for(nf)), dim3(256), 0, h->stream, fid_dev, h->node, nf,
-                       h->p.ni, h->p.nj, h->p.nk, h->p.halo_mode, nf, h->nbr, h->nbS);
-  HIP_OK(hipGetLastError());
-  HIP_OK(hipStreamSynchronize(h->stream));
-  HIP_OK(hipFree(fid_dev));
+  // fluid bitmask words with their compact base index, one row at a time
+  const int nx = h->p.ni;
+  const long rows = N / nx;
+  h->W = (nx + 63) / 64;
+  if (nx > 32767) return lbm_fail("LX above 32767 is not supported");
+  std::vector<RowWord> rw((size_t)rows * h->W);
+  fill_row_words(h->fid.data(), rows, nx, rw.data());
+  HIP_OK(hipMalloc(&h->rw, rw.size() * sizeof(RowWord)));
+  HIP_OK(hipMemcpy(h->rw, rw.data(), rw.size() * sizeof(RowWord), hipMemcpyHostToDevice));
 
   // faces along the last axis and the flux plane
   std::vector<int> lo, hi, fl;
@@ -548,11 +561,11 @@ static int transfer_f(lbm_handle *h, int subs, double *f1d, int up) {
     StepArgs<ST, CT> A = make_args<ST, CT>(h);
     dim3 g(blocks_for(nf)), b(256);
     if (up) {
-      if (subs == 0) hipLaunchKernelGGL((k_encode<QQ, ST, CT, 0>), g, b, 0, h->stream, A, dstage);
-      else hipLaunchKernelGGL((k_encode<QQ, ST, CT, 1>), g, b, 0, h->stream, A, dstage);
+      if (subs == 0) hipLaunchKernelGGL((k_encode<QQ, NS, ST, CT, 0>), g, b, 0, h->stream, A, dstage);
+      else if constexpr (NS == 2) hipLaunchKernelGGL((k_encode<QQ, NS, ST, CT, 1>), g, b, 0, h->stream, A, dstage);
     } else {
-      if (subs == 0) hipLaunchKernelGGL((k_decode<QQ, ST, CT, 0>), g, b, 0, h->stream, A, dstage);
-      else hipLaunchKernelGGL((k_decode<QQ, ST, CT, 1>), g, b, 0, h->stream, A, dstage);
+      if (subs == 0) hipLaunchKernelGGL((k_decode<QQ, NS, ST, CT, 0>), g, b, 0, h->stream, A, dstage);
+      else if constexpr (NS == 2) hipLaunchKernelGGL((k_decode<QQ, NS, ST, CT, 1>), g, b, 0, h->stream, A, dstage);
     }
     HIP_OK(hipGetLastError());
     return 0;
@@ -759,7 +772,7 @@ extern "C" int lbm_info(const lbm_handle *h, long *n_nodes, long *n_fluid, long 
   if (!h) return lbm_fail("null handle");
   if (n_nodes) *n_nodes = h->N;
   if (n_fluid) *n_fluid = h->nf;
-  if (row_stride) *row_stride = h->S;
+  if (row_stride) *row_stride = h->tiles * kTile;
   return 0;
 }
 
@@ -788,4 +801,36 @@ extern "C" int lbm_profile_read(lbm_handle *h, double *ms, int *count) {
     h->prof_n[c] = 0;
   }
   return 0;
+}
+
+extern "C" int lbm_copy_bandwidth(long nbytes, int reps, double *gbps) {
+  if (nbytes < 8 * 256 || reps < 1 || !gbps) return lbm_fail("lbm_copy_bandwidth: bad arguments");
+  long n = nbytes / 8;
+  double *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  float ms = 0.f;
+  if (hipMalloc(&a, n * 8) != hipSuccess || hipMalloc(&b, n * 8) != hipSuccess ||
+      hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    rc = lbm_fail("lbm_copy_bandwidth: allocation failed");
+  } else {
+    dim3 g(blocks_for(n)), bl(256);
+    (void)hipMemsetAsync(a, 0, n * 8, st);
+    hipLaunchKernelGGL(k_copy_probe, g, bl, 0, st, a, b, n);
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy_probe, g, bl, 0, st, a, b, n);
+    (void)hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0.f)
+      rc = lbm_fail("lbm_copy_bandwidth: timing failed");
+    else
+      *gbps = 2.0 * (double)n * 8.0 * reps / (ms * 1e-3) / 1e9;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
 }

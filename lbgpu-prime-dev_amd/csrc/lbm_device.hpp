@@ -1,15 +1,15 @@
 // Device code of the stream-collide library (gfx950, wave64).
 //
-// Storage: populations of fluid nodes only, SoA rows [subs][slot a][S]
-// indexed by the compact fluid index (dense order, i fastest).  Slot
-// meaning follows the reference's in-place AA scheme (Bailey et al., ICPP
-// 2009; src/k_stream_collide_stream.c, src/k_collide.c): before an odd
+// Storage: populations of fluid nodes only, in tiles of 64 consecutive
+// compact nodes (lbm_index.hpp), compact order = dense order (i fastest).
+// Slot meaning follows the reference's in-place AA scheme (Bailey et al.,
+// ICPP 2009; src/k_stream_collide_stream.c, src/k_collide.c): before an odd
 // half-step slot a at x holds the post-collision f_a(x); the odd half-step
 // pulls slot a from x - c_a, collides and pushes into slot opp(a) at
 // x + c_a; the even half-step reads the own node's opposite slots and writes
-// its own slots in order.  Halfway bounce-back is folded into a neighbour
-// table: nbr[a][x] is the compact index of x - c_a, or -1 when that node is
-// solid (then the own opposite slot is read / the own slot is written).
+// its own slots in order.  Halfway bounce-back: when x - c_a is solid the
+// pull reads the own slot opp(a); when x + c_a is solid the push writes the
+// own slot a (src/lattice.c:2730-2741, 2835-2844).
 //
 // All arithmetic is written in the reference's order; the library is built
 // with -ffp-contract=off so the FP64 path reproduces the FP64 oracle bit for
@@ -18,6 +18,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "lbm_index.hpp"
 
 namespace lbm {
 
@@ -65,12 +67,33 @@ struct Codec<double, float> {
   static __device__ __forceinline__ double enc(float v, float, float) { return (double)v; }
 };
 
+// Population element e of the tile array.  OFF32: the array is smaller
+// than 4 GiB, so the byte offset is a 32-bit value added to the uniform
+// base (saddr + voffset addressing, half the address registers).
+template <bool OFF32, typename T>
+__device__ __forceinline__ T *elem(T *F, uint32_t e) {
+  if (OFF32) return (T *)((char *)F + (uint32_t)(e * (uint32_t)sizeof(T)));
+  return F + e;
+}
+// Loads and stores of populations.  The even half-step streams whole
+// tiles and uses non-temporal accesses; the odd half-step's gathers and
+// scatters share partial lines between neighbouring waves, which must stay
+// in L2, so it uses ordinary ones.
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+  if (NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st_stream(T *p, T v) {
+  if (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 template <typename ST, typename CT>
 struct StepArgs {
-  ST *F;          // [nsubs][Q][S]
-  long S;         // slots per direction row
-  const int *nbr; // [Q-1][nbS]
-  long nbS;
+  ST *F;          // tiles [nf/64][NS][Q][64]
+  Geo G;
   int nf;         // fluid nodes
   int nblk8;      // launched blocks / 8
   double *mv;     // [nsubs][1+D][nf]
@@ -94,38 +117,51 @@ __device__ __forceinline__ CT shift_of(const StepArgs<ST, CT> &A, int a) {
   return SUBS == 1 ? (CT)W<Q>(a) * A.c0 : (CT)0;
 }
 
+// nb[a] = compact index of x - c_a (or -1 when solid), a = 1..Q-1
+template <int Q>
+__device__ __forceinline__ void neighbors(const Geo &G, int x, int (&nb)[Q]) {
+  constexpr int D = Q == 9 ? 2 : 3;
+  int i, j, k;
+  coords<D>(G, (uint32_t)G.node[x], i, j, k);
+  int r[Q - 1];
+  neighbors_rows<D>(G, i, j, k, r);
+  nb[0] = x;
+#pragma unroll
+  for (int a = 1; a < Q; a++) nb[a] = r[a - 1];
+}
+
 // Gather the Q populations one half-step collides.
-template <int Q, typename ST, typename CT, int SUBS, bool ODD>
+template <int Q, int NS, typename ST, typename CT, int SUBS, bool ODD, bool OFF32 = false>
 __device__ __forceinline__ void gather(const StepArgs<ST, CT> &A, int idx, const int (&nb)[Q],
                                        CT (&f)[Q]) {
-  const ST *Fs = A.F + (long)SUBS * Q * A.S;
 #pragma unroll
   for (int a = 0; a < Q; a++) {
-    long off;
+    uint32_t off;
     if (a == 0)
-      off = idx;
+      off = foff<Q, NS>(SUBS, 0, idx);
     else if (ODD)
-      off = nb[a] >= 0 ? (long)a * A.S + nb[a] : (long)OPP(a) * A.S + idx;
+      off = nb[a] >= 0 ? foff<Q, NS>(SUBS, a, nb[a]) : foff<Q, NS>(SUBS, OPP(a), idx);
     else
-      off = (long)OPP(a) * A.S + idx;
-    f[a] = Codec<ST, CT>::dec(Fs[off], A.inv_scale[SUBS], shift_of<Q, ST, CT, SUBS>(A, a));
+      off = foff<Q, NS>(SUBS, OPP(a), idx);
+    f[a] = Codec<ST, CT>::dec(ld_stream<!ODD>(elem<OFF32>(A.F, off)),
+                              A.inv_scale[SUBS], shift_of<Q, ST, CT, SUBS>(A, a));
   }
 }
 
-template <int Q, typename ST, typename CT, int SUBS, bool ODD>
+template <int Q, int NS, typename ST, typename CT, int SUBS, bool ODD, bool OFF32 = false>
 __device__ __forceinline__ void scatter(const StepArgs<ST, CT> &A, int idx, const int (&nb)[Q],
                                         const CT (&f)[Q]) {
-  ST *Fs = A.F + (long)SUBS * Q * A.S;
 #pragma unroll
   for (int a = 0; a < Q; a++) {
-    long off;
+    uint32_t off;
     if (a == 0)
-      off = idx;
+      off = foff<Q, NS>(SUBS, 0, idx);
     else if (ODD)
-      off = nb[OPP(a)] >= 0 ? (long)OPP(a) * A.S + nb[OPP(a)] : (long)a * A.S + idx;
+      off = nb[OPP(a)] >= 0 ? foff<Q, NS>(SUBS, OPP(a), nb[OPP(a)]) : foff<Q, NS>(SUBS, a, idx);
     else
-      off = (long)a * A.S + idx;
-    Fs[off] = Codec<ST, CT>::enc(f[a], A.scale[SUBS], shift_of<Q, ST, CT, SUBS>(A, a));
+      off = foff<Q, NS>(SUBS, a, idx);
+    st_stream<!ODD>(
+        elem<OFF32>(A.F, off), Codec<ST, CT>::enc(f[a], A.scale[SUBS], shift_of<Q, ST, CT, SUBS>(A, a)));
   }
 }
 
@@ -188,24 +224,21 @@ __device__ __forceinline__ void collide(const StepArgs<ST, CT> &A, int idx, CT (
   }
 }
 
-template <int Q, int NS, typename ST, typename CT, bool ODD>
+template <int Q, int NS, typename ST, typename CT, bool ODD, bool OFF32>
 __global__ __launch_bounds__(kBlock) void k_half_step(StepArgs<ST, CT> A) {
   const int idx = xcd_block(A.nblk8) * kBlock + (int)threadIdx.x;
   if (idx >= A.nf) return;
   int nb[Q];
   nb[0] = idx;
-  if (ODD) {
-#pragma unroll
-    for (int a = 1; a < Q; a++) nb[a] = A.nbr[(long)(a - 1) * A.nbS + idx];
-  }
+  if (ODD) neighbors<Q>(A.G, idx, nb);
   CT f[Q], u[3];
-  gather<Q, ST, CT, 0, ODD>(A, idx, nb, f);
+  gather<Q, NS, ST, CT, 0, ODD, OFF32>(A, idx, nb, f);
   collide<Q, ST, CT, 0, ODD>(A, idx, f, u);
-  scatter<Q, ST, CT, 0, ODD>(A, idx, nb, f);
+  scatter<Q, NS, ST, CT, 0, ODD, OFF32>(A, idx, nb, f);
   if (NS == 2 && A.solute_on) {
-    gather<Q, ST, CT, 1, ODD>(A, idx, nb, f);
+    gather<Q, NS, ST, CT, 1, ODD, OFF32>(A, idx, nb, f);
     collide<Q, ST, CT, 1, ODD>(A, idx, f, u);
-    scatter<Q, ST, CT, 1, ODD>(A, idx, nb, f);
+    scatter<Q, NS, ST, CT, 1, ODD, OFF32>(A, idx, nb, f);
   }
 }
 
@@ -219,15 +252,14 @@ struct BcArgs {
   int phase;        // 1: before the odd half-step, 2: before the even one
 };
 
-template <int Q, typename ST, typename CT, int SUBS>
+template <int Q, int NS, typename ST, typename CT, int SUBS>
 __device__ __forceinline__ void bc_gather(const BcArgs<ST, CT> &B, int idx, int (&nb)[Q], CT (&f)[Q]) {
   nb[0] = idx;
   if (B.phase == 1) {
-#pragma unroll
-    for (int a = 1; a < Q; a++) nb[a] = B.A.nbr[(long)(a - 1) * B.A.nbS + idx];
-    gather<Q, ST, CT, SUBS, true>(B.A, idx, nb, f);
+    neighbors<Q>(B.A.G, idx, nb);
+    gather<Q, NS, ST, CT, SUBS, true>(B.A, idx, nb, f);
   } else {
-    gather<Q, ST, CT, SUBS, false>(B.A, idx, nb, f);
+    gather<Q, NS, ST, CT, SUBS, false>(B.A, idx, nb, f);
   }
 }
 
@@ -236,28 +268,27 @@ __device__ __forceinline__ void bc_gather(const BcArgs<ST, CT> &B, int idx, int 
 //   solid the reference writes the own slot a (set_f1d_d, da = 0,
 //   src/lattice.c:2835-2844) unless own_opp, which writes the own slot
 //   opp(a) that the pull then reads;  phase 2 -> own slot opp(a).
-template <int Q, typename ST, typename CT, int SUBS>
+template <int Q, int NS, typename ST, typename CT, int SUBS>
 __device__ __forceinline__ void bc_store(const BcArgs<ST, CT> &B, int idx, const int (&nb)[Q], int a,
                                          CT v, bool own_opp) {
-  ST *Fs = B.A.F + (long)SUBS * Q * B.A.S;
-  long off;
+  uint32_t off;
   if (B.phase == 1)
-    off = nb[a] >= 0 ? (long)a * B.A.S + nb[a] : (long)(own_opp ? OPP(a) : a) * B.A.S + idx;
+    off = nb[a] >= 0 ? foff<Q, NS>(SUBS, a, nb[a]) : foff<Q, NS>(SUBS, own_opp ? OPP(a) : a, idx);
   else
-    off = (long)OPP(a) * B.A.S + idx;
-  Fs[off] = Codec<ST, CT>::enc(v, B.A.scale[SUBS], shift_of<Q, ST, CT, SUBS>(B.A, a));
+    off = foff<Q, NS>(SUBS, OPP(a), idx);
+  B.A.F[off] = Codec<ST, CT>::enc(v, B.A.scale[SUBS], shift_of<Q, ST, CT, SUBS>(B.A, a));
 }
 
 // 2-D Zou-He pressure on the N (j = nj-1) or S (j = 0) row:
 // k_sysbound_pressure_{n,s}_{1,2}, src/system_boundary_kernels.c:4-451
-template <typename ST, typename CT, int SUBS>
+template <int NS, typename ST, typename CT, int SUBS>
 __global__ void k_bc2d_pressure(BcArgs<ST, CT> B, int north, CT rho_b) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= B.n) return;
   const int idx = B.list[t];
   int nb[9];
   CT f[9];
-  bc_gather<9, ST, CT, SUBS>(B, idx, nb, f);
+  bc_gather<9, NS, ST, CT, SUBS>(B, idx, nb, f);
   enum { C_ = 0, E_, W_, N_, S_, NE_, SW_, NW_, SE_ };
   const CT rhoA = SUBS == 0 ? B.A.rhoA : B.A.rhoA_s;
   if (north) {
@@ -267,9 +298,9 @@ __global__ void k_bc2d_pressure(BcArgs<ST, CT> B, int north, CT rho_b) {
     CT vS = f[N_] - ((CT)(2. / 3.)) * x;
     CT vSW = f[NE_] + ((CT)(1. / 2.)) * (f[E_] - f[W_]) - ((CT)(1. / 6.)) * x;
     CT vSE = f[NW_] + ((CT)(1. / 2.)) * (f[W_] - f[E_]) - ((CT)(1. / 6.)) * x;
-    bc_store<9, ST, CT, SUBS>(B, idx, nb, S_, vS, false);
-    bc_store<9, ST, CT, SUBS>(B, idx, nb, SE_, vSE, false);
-    bc_store<9, ST, CT, SUBS>(B, idx, nb, SW_, vSW, false);
+    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, S_, vS, false);
+    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, SE_, vSE, false);
+    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, SW_, vSW, false);
   } else {
     CT x = -f[C_] - f[E_] - f[W_] - f[S_] - f[S_] - f[SE_] - f[SE_] - f[SW_] - f[SW_];
     x += rho_b;
@@ -277,9 +308,9 @@ __global__ void k_bc2d_pressure(BcArgs<ST, CT> B, int north, CT rho_b) {
     CT vN = f[S_] + ((CT)(2. / 3.)) * x;
     CT vNE = f[SW_] + ((CT)(1. / 2.)) * (f[W_] - f[E_]) + ((CT)(1. / 6.)) * x;
     CT vNW = f[SE_] + ((CT)(1. / 2.)) * (f[E_] - f[W_]) + ((CT)(1. / 6.)) * x;
-    bc_store<9, ST, CT, SUBS>(B, idx, nb, N_, vN, false);
-    bc_store<9, ST, CT, SUBS>(B, idx, nb, NW_, vNW, false);
-    bc_store<9, ST, CT, SUBS>(B, idx, nb, NE_, vNE, false);
+    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, N_, vN, false);
+    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, NW_, vNW, false);
+    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, NE_, vNE, false);
   }
 }
 
@@ -292,19 +323,19 @@ __global__ void k_bc2d_zerograd(BcArgs<ST, CT> B, int north) {
   const int idx = B.list[t];
   int nb[9];
   CT f[9];
-  bc_gather<9, ST, CT, 1>(B, idx, nb, f);
+  bc_gather<9, 2, ST, CT, 1>(B, idx, nb, f);
   enum { C_ = 0, E_, W_, N_, S_, NE_, SW_, NW_, SE_ };
   constexpr double w1 = W<9>(1), w2 = W<9>(5);
   if (north) {
     CT c = (f[N_] + f[NE_] + f[NW_]) / ((CT)(w1 + w2 + w2));
-    bc_store<9, ST, CT, 1>(B, idx, nb, S_, (CT)w1 * c, false);
-    bc_store<9, ST, CT, 1>(B, idx, nb, SE_, (CT)w2 * c, false);
-    bc_store<9, ST, CT, 1>(B, idx, nb, SW_, (CT)w2 * c, false);
+    bc_store<9, 2, ST, CT, 1>(B, idx, nb, S_, (CT)w1 * c, false);
+    bc_store<9, 2, ST, CT, 1>(B, idx, nb, SE_, (CT)w2 * c, false);
+    bc_store<9, 2, ST, CT, 1>(B, idx, nb, SW_, (CT)w2 * c, false);
   } else {
     CT c = (f[S_] + f[SW_] + f[SE_]) / ((CT)(w1 + w2 + w2));
-    bc_store<9, ST, CT, 1>(B, idx, nb, N_, (CT)w1 * c, false);
-    bc_store<9, ST, CT, 1>(B, idx, nb, NW_, (CT)w2 * c, false);
-    bc_store<9, ST, CT, 1>(B, idx, nb, NE_, (CT)w2 * c, false);
+    bc_store<9, 2, ST, CT, 1>(B, idx, nb, N_, (CT)w1 * c, false);
+    bc_store<9, 2, ST, CT, 1>(B, idx, nb, NW_, (CT)w2 * c, false);
+    bc_store<9, 2, ST, CT, 1>(B, idx, nb, NE_, (CT)w2 * c, false);
   }
 }
 
@@ -319,10 +350,9 @@ struct ZfaceArgs {
   double rho_b;       // prescribed density of this face
   const double *cmap; // [nj][ni] inlet concentration or null
   double c_b;
-  const int *node;    // compact -> dense index
 };
 
-template <typename ST, typename CT, int NS>
+template <int NS, typename ST, typename CT>
 __global__ void k_bc_zface(BcArgs<ST, CT> B, ZfaceArgs Z) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= B.n) return;
@@ -332,7 +362,7 @@ __global__ void k_bc_zface(BcArgs<ST, CT> B, ZfaceArgs Z) {
   int nb[19];
   CT f[19];
   if (Z.flow) {
-    bc_gather<19, ST, CT, 0>(B, idx, nb, f);
+    bc_gather<19, NS, ST, CT, 0>(B, idx, nb, f);
     CT P = f[0];
 #pragma unroll
     for (int a = 1; a < 9; a++) P += f[a];
@@ -344,39 +374,39 @@ __global__ void k_bc_zface(BcArgs<ST, CT> B, ZfaceArgs Z) {
     CT drb = (CT)Z.rho_b - B.A.rhoA;
     if (!Z.top) {
       CT J = drb - (P + (CT)2 * M);
-      bc_store<19, ST, CT, 0>(B, idx, nb, 9, f[10] + J / (CT)3, true);
-      bc_store<19, ST, CT, 0>(B, idx, nb, 11, f[12] + J / (CT)6 - Nx, true);
-      bc_store<19, ST, CT, 0>(B, idx, nb, 13, f[14] + J / (CT)6 + Nx, true);
-      bc_store<19, ST, CT, 0>(B, idx, nb, 15, f[16] + J / (CT)6 - Ny, true);
-      bc_store<19, ST, CT, 0>(B, idx, nb, 17, f[18] + J / (CT)6 + Ny, true);
+      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 9, f[10] + J / (CT)3, true);
+      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 11, f[12] + J / (CT)6 - Nx, true);
+      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 13, f[14] + J / (CT)6 + Nx, true);
+      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 15, f[16] + J / (CT)6 - Ny, true);
+      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 17, f[18] + J / (CT)6 + Ny, true);
     } else {
       CT J = (P + (CT)2 * M) - drb;
-      bc_store<19, ST, CT, 0>(B, idx, nb, 10, f[9] - J / (CT)3, true);
-      bc_store<19, ST, CT, 0>(B, idx, nb, 12, f[11] - J / (CT)6 + Nx, true);
-      bc_store<19, ST, CT, 0>(B, idx, nb, 14, f[13] - J / (CT)6 - Nx, true);
-      bc_store<19, ST, CT, 0>(B, idx, nb, 16, f[15] - J / (CT)6 + Ny, true);
-      bc_store<19, ST, CT, 0>(B, idx, nb, 18, f[17] - J / (CT)6 - Ny, true);
+      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 10, f[9] - J / (CT)3, true);
+      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 12, f[11] - J / (CT)6 + Nx, true);
+      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 14, f[13] - J / (CT)6 - Nx, true);
+      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 16, f[15] - J / (CT)6 + Ny, true);
+      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 18, f[17] - J / (CT)6 - Ny, true);
     }
   }
   if (NS == 2 && Z.solute) {
-    bc_gather<19, ST, CT, 1>(B, idx, nb, f);
+    bc_gather<19, NS, ST, CT, 1>(B, idx, nb, f);
     CT known = (CT)0, cs;
     if (!Z.top) {
       CT cb = (CT)Z.c_b;
-      if (Z.cmap) cb = (CT)Z.cmap[Z.node[idx]];  // z = 0: dense index = i + ni j
+      if (Z.cmap) cb = (CT)Z.cmap[B.A.G.node[idx]];  // z = 0: dense index = i + ni j
 #pragma unroll
       for (int a = 0; a < 9; a++) known += f[a];
 #pragma unroll
       for (int q = 0; q < 5; q++) known += f[DN[q]];
       cs = (cb - known) / (CT)(1. / 6.);
 #pragma unroll
-      for (int q = 0; q < 5; q++) bc_store<19, ST, CT, 1>(B, idx, nb, UP[q], (CT)W<19>(UP[q]) * cs, true);
+      for (int q = 0; q < 5; q++) bc_store<19, NS, ST, CT, 1>(B, idx, nb, UP[q], (CT)W<19>(UP[q]) * cs, true);
     } else {
 #pragma unroll
       for (int q = 0; q < 5; q++) known += f[UP[q]];
       cs = known / (CT)(1. / 6.);
 #pragma unroll
-      for (int q = 0; q < 5; q++) bc_store<19, ST, CT, 1>(B, idx, nb, DN[q], (CT)W<19>(DN[q]) * cs, true);
+      for (int q = 0; q < 5; q++) bc_store<19, NS, ST, CT, 1>(B, idx, nb, DN[q], (CT)W<19>(DN[q]) * cs, true);
     }
   }
 }
