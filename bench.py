@@ -9,8 +9,9 @@ half-step, each with its boundary kernels) over the whole lattice, with all
 populations resident in HBM.
 
 Also measured in the same run and reported under "extra": D3Q19 flow +
-solute at 256^3 in FP64, FP32-shifted and FP16-scaled storage (FP32
-collision), and the reference-style CPU oracle on a bounded sample.
+solute at 256^3 in FP64, FP32-shifted (FP32 or FP64 collision) and
+FP16-scaled storage (FP32 collision), a device copy bandwidth, and the CPU
+oracle on a bounded sample of the default workload (cpu_baseline).
 """
 import argparse
 import importlib.util
@@ -161,7 +162,7 @@ def main():
 
     if not args.no_extra:
         variants = [("flow_solute_fp64", 0, 0, 8), ("flow_solute_fp32_shifted", 1, 1, 4),
-                    ("flow_solute_fp16_scaled", 2, 1, 2)]
+                    ("flow_solute_fp32_shifted_f64_collision", 1, 0, 4), ("flow_solute_fp16_scaled", 2, 1, 2)]
         ext = {}
         for name, st, ct, s in variants:
             kw2 = dict(base, nsubs=2, zface_solute=1, c_bot=1.0, storage=st, compute=ct,

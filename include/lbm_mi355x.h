@@ -156,6 +156,20 @@ int lbm_info(const lbm_handle *h, long *n_nodes, long *n_fluid, long *row_stride
 int lbm_profile_enable(lbm_handle *h, int on);
 int lbm_profile_read(lbm_handle *h, double *ms, int *count);
 
+/* Objective of the discrete adjoint (no reference counterpart): runs
+ * nsteps steps from the current state like lbm_run, keeping a copy of the
+ * state every `every` steps, and returns
+ *   J = 1/(window * LX * LY) * sum over the last `window` steps of
+ *       sum over the fluid nodes of plane flux_plane of sum_a c_a,z g_a
+ * (the solute flux through that plane, pre-collision, after the BCs).
+ * D3Q19 flow + solute, FP64, halo_mode 0, sink 0. */
+int lbm_objective_run(lbm_handle *h, int nsteps, int window, int every, double *J);
+/* Gradient of the last objective by the discrete adjoint, replaying each
+ * segment between saved states out of place and sweeping it backwards:
+ * grad[0..4] = dJ/d(g_x, g_y, g_z, tau, tau_s) of the flow's body force
+ * and relaxation times; grad_cmap (optional, [LY][LX]) = dJ/dc_bot_map. */
+int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap);
+
 /* Device-to-device copy of nbytes (reps times) on a stream of its own:
  * the measured bandwidth in GB/s (bytes read + written per second) is the
  * practical ceiling that the half-steps are compared with. */

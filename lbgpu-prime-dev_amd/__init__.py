@@ -92,6 +92,8 @@ def lib():
             "lbm_profile_enable": [vp, ip],
             "lbm_profile_read": [vp, dp, ctypes.POINTER(ctypes.c_int)],
             "lbm_copy_bandwidth": [ctypes.c_long, ip, dp],
+            "lbm_objective_run": [vp, ip, ip, ip, dp],
+            "lbm_adjoint_run": [vp, dp, dp],
             "lbm_geom_discs2d": [ip, ip, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_double, ip, u8p, dp],
             "lbm_geom_spheres3d": [ip, ip, ip, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
@@ -290,6 +292,19 @@ class Lattice:
         a, b = ctypes.c_double(), ctypes.c_double()
         _check(lib().lbm_last_flux(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def objective(self, nsteps, window, every=100):
+        """J over the last `window` of `nsteps` steps (see lbm_objective_run)"""
+        J = ctypes.c_double()
+        _check(lib().lbm_objective_run(self.h, int(nsteps), int(window), int(every), ctypes.byref(J)))
+        return J.value
+
+    def adjoint(self, with_map=True):
+        """(dJ/d(gx, gy, gz, tau, tau_s), dJ/dc_bot_map or None)"""
+        g = np.empty(5)
+        m = np.empty((self.p.nj, self.p.ni)) if with_map else None
+        _check(lib().lbm_adjoint_run(self.h, _dp(g), _dp(m) if m is not None else None))
+        return g, m
 
     def profile(self, on=True):
         _check(lib().lbm_profile_enable(self.h, int(bool(on))))

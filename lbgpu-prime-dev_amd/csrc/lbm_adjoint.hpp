@@ -1,0 +1,244 @@
+// Discrete adjoint of the D3Q19 flow + solute time step (FP64, periodic
+// lattice, z-face boundary conditions).  No reference counterpart.
+//
+// A time step is  state_t = K_t(B_t(state_{t-1}))  with B_t the z-face
+// conditions and K_t the half-step kernel.  K_t gathers every stored slot
+// exactly once, collides per node and scatters every slot exactly once, so
+// its transpose reads the adjoint at the places K_t wrote, applies the
+// transposed collision Jacobian at the forward state, and writes at the
+// places K_t read - in place and race-free for the same reason as the
+// forward half-step.  B_t overwrote the unknown populations of the face
+// nodes with linear combinations of the known ones; its transpose adds the
+// unknowns' adjoint to the known ones and clears the unknowns.
+//
+// Objective: J = 1/(W A) sum over the last W steps of the solute flux
+// sum_a c_a,z g_a through plane zf (pre-collision, after the BCs), A =
+// LX*LY.  Parameters: body force g, tau, tau_s and the inlet concentration
+// map c_b(x, y).
+#pragma once
+
+#include "lbm_aux.hpp"
+
+namespace lbm {
+
+struct AdjArgs {
+  StepArgs<double, double> A;  // A.F: forward state before the step, after its BCs
+  double *L;                   // adjoint state, same tile layout as the populations
+  double seed;                 // dJ / d(plane flux of this step); 0 outside the window
+  int zf;                      // flux plane
+  double *gacc;                // [launched blocks][5]: dJ/dg_x, g_y, g_z, tau, tau_s
+};
+
+// slot offset a half-step reads for direction a (the pull) / writes (the push)
+template <bool ODD>
+__device__ __forceinline__ uint32_t pull_off(const int (&nb)[19], int s, int a, int idx) {
+  if (a == 0) return foff<19, 2>(s, 0, idx);
+  if (ODD) return nb[a] >= 0 ? foff<19, 2>(s, a, nb[a]) : foff<19, 2>(s, OPP(a), idx);
+  return foff<19, 2>(s, OPP(a), idx);
+}
+template <bool ODD>
+__device__ __forceinline__ uint32_t push_off(const int (&nb)[19], int s, int a, int idx) {
+  if (a == 0) return foff<19, 2>(s, 0, idx);
+  if (ODD) return nb[OPP(a)] >= 0 ? foff<19, 2>(s, OPP(a), nb[OPP(a)]) : foff<19, 2>(s, a, idx);
+  return foff<19, 2>(s, a, idx);
+}
+
+template <bool ODD, bool OFF32>
+__global__ __launch_bounds__(kBlock) void k_adjoint_step(AdjArgs J) {
+  constexpr int Q = 19;
+  const int lb = xcd_block(J.A.nblk8);
+  const int idx = lb * kBlock + (int)threadIdx.x;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (idx < J.A.nf) {
+    const StepArgs<double, double> &A = J.A;
+    int nb[Q];
+    nb[0] = idx;
+    int i, j, k;
+    coords<3>(A.G, (uint32_t)A.G.node[idx], i, j, k);
+    if (ODD) {
+      int r[18];
+      neighbors_rows<3>(A.G, i, j, k, r);
+#pragma unroll
+      for (int a = 1; a < Q; a++) nb[a] = r[a - 1];
+    }
+    double f[Q], g[Q], lam[Q], mu[Q];
+    gather<Q, 2, double, double, 0, ODD, OFF32>(A, idx, nb, f);
+    gather<Q, 2, double, double, 1, ODD, OFF32>(A, idx, nb, g);
+#pragma unroll
+    for (int a = 0; a < Q; a++) {
+      lam[a] = J.L[push_off<ODD>(nb, 0, a, idx)];
+      mu[a] = J.L[push_off<ODD>(nb, 1, a, idx)];
+    }
+    const double rhoA = A.rhoA, tau = A.tau, om = A.inv_tau, oms = A.inv_tau_s;
+    double drho = 0.0, u[3] = {0.0, 0.0, 0.0}, c = 0.0;
+#pragma unroll
+    for (int a = 0; a < Q; a++) {
+      drho += f[a];
+      u[0] += CX(a) * f[a];
+      u[1] += CY(a) * f[a];
+      u[2] += CZ(a) * f[a];
+      c += g[a];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) u[d] = u[d] / rhoA + tau * A.g[d];
+    const double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    // solute: g* = (1 - ws) g + ws w c (1 + 3 c.u)
+    double Sg = 0.0, Mu[3] = {0.0, 0.0, 0.0}, dts = 0.0;
+    // flow: f* = (1 - w) f + w feq(drho, u)
+    double Lr = 0.0, Lu[3] = {0.0, 0.0, 0.0}, dt = 0.0;
+#pragma unroll
+    for (int a = 0; a < Q; a++) {
+      const double w = W<19>(a);
+      const double cu = CX(a) * u[0] + CY(a) * u[1] + CZ(a) * u[2];
+      const double geq = w * c * (1.0 + 3.0 * cu);
+      Sg += mu[a] * w * (1.0 + 3.0 * cu);
+      Mu[0] += mu[a] * w * CX(a);
+      Mu[1] += mu[a] * w * CY(a);
+      Mu[2] += mu[a] * w * CZ(a);
+      dts += mu[a] * (geq - g[a]);
+      const double feq = w * (drho + rhoA * (3.0 * cu + 4.5 * cu * cu - 1.5 * usq));
+      Lr += w * lam[a];
+      const double t9 = lam[a] * w * rhoA;
+      Lu[0] += t9 * (3.0 * CX(a) + 9.0 * cu * CX(a) - 3.0 * u[0]);
+      Lu[1] += t9 * (3.0 * CY(a) + 9.0 * cu * CY(a) - 3.0 * u[1]);
+      Lu[2] += t9 * (3.0 * CZ(a) + 9.0 * cu * CZ(a) - 3.0 * u[2]);
+      dt += lam[a] * (feq - f[a]);
+    }
+    double G[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) G[d] = om * Lu[d] + 3.0 * oms * c * Mu[d];
+    const double seed = k == J.zf ? J.seed : 0.0;
+#pragma unroll
+    for (int b = 0; b < Q; b++) {
+      const double lb_ = (1.0 - om) * lam[b] + om * Lr + (CX(b) * G[0] + CY(b) * G[1] + CZ(b) * G[2]) / rhoA;
+      const double mb = (1.0 - oms) * mu[b] + oms * Sg + seed * CZ(b);
+      J.L[pull_off<ODD>(nb, 0, b, idx)] = lb_;
+      J.L[pull_off<ODD>(nb, 1, b, idx)] = mb;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) acc[d] = tau * G[d];
+    acc[3] = -om * om * dt + A.g[0] * G[0] + A.g[1] * G[1] + A.g[2] * G[2];
+    acc[4] = -oms * oms * dts;
+  }
+  __shared__ double red[5];
+  const int kinds[5] = {0, 0, 0, 0, 0};
+  block_reduce<5>(acc, kinds, red);
+  if (threadIdx.x < 5) J.gacc[(long)lb * 5 + threadIdx.x] += red[threadIdx.x];
+}
+
+// Transpose of the z-face conditions of k_bc_zface for one face.
+struct AdjBcArgs {
+  StepArgs<double, double> A;
+  double *L;
+  const int *list;
+  int n;
+  int phase;
+};
+
+template <bool ODD>
+__device__ __forceinline__ void adj_face(double *L, const int (&nb)[19], int s, int idx, bool top, bool flow,
+                                        double &dcb) {
+  constexpr int UP[5] = {9, 11, 13, 15, 17};
+  constexpr int DN[5] = {10, 12, 14, 16, 18};
+  uint32_t loc[19];
+#pragma unroll
+  for (int a = 0; a < 19; a++) loc[a] = pull_off<ODD>(nb, s, a, idx);
+  double lu[5], dl[19];
+#pragma unroll
+  for (int a = 0; a < 19; a++) dl[a] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 5; q++) lu[q] = L[loc[top ? DN[q] : UP[q]]];
+  if (flow) {
+    double lJ, lNx, lNy;
+    if (!top) {
+      // J = drb - (P + 2M); v9 = f10 + J/3, v11 = f12 + J/6 - Nx, v13 = f14 + J/6 + Nx,
+      // v15 = f16 + J/6 - Ny, v17 = f18 + J/6 + Ny
+      lJ = lu[0] / 3.0 + (lu[1] + lu[2] + lu[3] + lu[4]) / 6.0;
+      lNx = -lu[1] + lu[2];
+      lNy = -lu[3] + lu[4];
+#pragma unroll
+      for (int q = 0; q < 5; q++) dl[DN[q]] += lu[q] - 2.0 * lJ;
+#pragma unroll
+      for (int a = 0; a < 9; a++) dl[a] -= lJ;
+    } else {
+      // J = (P + 2M) - drb; v10 = f9 - J/3, v12 = f11 - J/6 + Nx, v14 = f13 - J/6 - Nx,
+      // v16 = f15 - J/6 + Ny, v18 = f17 - J/6 - Ny
+      lJ = -lu[0] / 3.0 - (lu[1] + lu[2] + lu[3] + lu[4]) / 6.0;
+      lNx = lu[1] - lu[2];
+      lNy = lu[3] - lu[4];
+#pragma unroll
+      for (int q = 0; q < 5; q++) dl[UP[q]] += lu[q] + 2.0 * lJ;
+#pragma unroll
+      for (int a = 0; a < 9; a++) dl[a] += lJ;
+    }
+    // Nx = ((f1 + f5 + f8) - (f2 + f7 + f6)) / 2, Ny = ((f3 + f5 + f7) - (f4 + f6 + f8)) / 2
+    dl[1] += lNx / 2.0;
+    dl[5] += lNx / 2.0;
+    dl[8] += lNx / 2.0;
+    dl[2] -= lNx / 2.0;
+    dl[7] -= lNx / 2.0;
+    dl[6] -= lNx / 2.0;
+    dl[3] += lNy / 2.0;
+    dl[5] += lNy / 2.0;
+    dl[7] += lNy / 2.0;
+    dl[4] -= lNy / 2.0;
+    dl[6] -= lNy / 2.0;
+    dl[8] -= lNy / 2.0;
+  } else {
+    // solute: unknown v_u = w_u c*; bottom c* = (c_b - known) / (1/6), top c* = sum(+z) / (1/6)
+    double lcs = 0.0;
+#pragma unroll
+    for (int q = 0; q < 5; q++) lcs += W<19>(top ? DN[q] : UP[q]) * lu[q];
+    const double k6 = lcs / (1. / 6.);
+    if (!top) {
+#pragma unroll
+      for (int a = 0; a < 9; a++) dl[a] -= k6;
+#pragma unroll
+      for (int q = 0; q < 5; q++) dl[DN[q]] -= k6;
+      dcb = k6;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; q++) dl[UP[q]] += k6;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 19; a++) {
+    bool unknown = false;
+#pragma unroll
+    for (int q = 0; q < 5; q++) unknown |= a == (top ? DN[q] : UP[q]);
+    if (unknown)
+      L[loc[a]] = 0.0;
+    else if (dl[a] != 0.0)
+      L[loc[a]] += dl[a];
+  }
+}
+
+__global__ void k_adjoint_zface(AdjBcArgs B, ZfaceArgs Z, double *cgrad) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B.n) return;
+  const int idx = B.list[t];
+  int nb[19];
+  nb[0] = idx;
+  double dcb = 0.0;
+  if (B.phase == 1) {
+    int i, j, k;
+    coords<3>(B.A.G, (uint32_t)B.A.G.node[idx], i, j, k);
+    int r[18];
+    neighbors_rows<3>(B.A.G, i, j, k, r);
+#pragma unroll
+    for (int a = 1; a < 19; a++) nb[a] = r[a - 1];
+    if (Z.flow) adj_face<true>(B.L, nb, 0, idx, Z.top, true, dcb);
+    if (Z.solute) adj_face<true>(B.L, nb, 1, idx, Z.top, false, dcb);
+  } else {
+    if (Z.flow) adj_face<false>(B.L, nb, 0, idx, Z.top, true, dcb);
+    if (Z.solute) adj_face<false>(B.L, nb, 1, idx, Z.top, false, dcb);
+  }
+  if (!Z.top && Z.solute && cgrad) cgrad[B.A.G.node[idx]] += dcb;
+}
+
+// running sum of the plane flux over the objective window
+__global__ void k_accumulate(const double *v, double *acc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) acc[0] += v[0];
+}
+
+}  // namespace lbm

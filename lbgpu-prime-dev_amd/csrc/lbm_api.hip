@@ -6,9 +6,11 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
+#include "lbm_adjoint.hpp"
 #include "lbm_aux.hpp"
 #include "lbm_device.hpp"
 #include "lbm_internal.h"
@@ -81,6 +83,14 @@ struct lbm_handle {
   std::vector<hipEvent_t> pool;
   double prof_ms[5] = {0, 0, 0, 0, 0};
   int prof_n[5] = {0, 0, 0, 0, 0};
+  // objective run and its adjoint
+  struct {
+    int valid = 0;
+    int t0 = 0, nsteps = 0, window = 0, every = 0;
+    std::vector<void *> ckpt;  // state before steps t0+1, t0+every+1, ...
+    double *jacc = nullptr;
+    double J = 0.0;
+  } obj;
 };
 
 namespace {
@@ -124,6 +134,7 @@ StepArgs<ST, CT> make_args(const lbm_handle *h) {
   StepArgs<ST, CT> A;
   const lbm_params &p = h->p;
   A.F = (ST *)h->F;
+  A.Fo = (ST *)h->F;
   A.G.rw = h->rw;
   A.G.node = h->node;
   A.G.ni = p.ni;
@@ -181,7 +192,17 @@ int dispatch(const lbm_handle *h, Fn &&fn) {
 
 inline unsigned blocks_for(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
+void free_objective(lbm_handle *h) {
+  for (void *q : h->obj.ckpt)
+    if (q) (void)hipFree(q);
+  h->obj.ckpt.clear();
+  if (h->obj.jacc) (void)hipFree(h->obj.jacc);
+  h->obj.jacc = nullptr;
+  h->obj.valid = 0;
+}
+
 void free_lattice(lbm_handle *h) {
+  free_objective(h);
   void *ptrs[] = {h->F, h->rw, h->mv, h->node, h->face_lo, h->face_hi, h->face_flux, h->plane_lo,
                   h->plane_hi, h->tgt_lo, h->tgt_hi, h->halo_pos, h->halo_neg};
   for (void *q : ptrs)
@@ -365,7 +386,7 @@ int launch_flux(lbm_handle *h, int phase) {
   });
 }
 
-int do_step(lbm_handle *h, int write_mv) {
+int do_step(lbm_handle *h, int write_mv, double *flux_acc = nullptr) {
   int t = h->time + 1;
   int odd = t & 1;
   int phase = odd ? 1 : 2;
@@ -378,6 +399,10 @@ int do_step(lbm_handle *h, int write_mv) {
   if (h->n_flux && sol_on) {
     if (launch_flux(h, phase)) return 1;
     h->flux_valid = 1;
+    if (flux_acc) {
+      hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(64), 0, h->stream, h->flux_dev, flux_acc);
+      HIP_OK(hipGetLastError());
+    }
   }
   if (launch_half(h, odd, t, write_mv)) return 1;
   h->time = t;
@@ -832,5 +857,185 @@ extern "C" int lbm_copy_bandwidth(long nbytes, int reps, double *gbps) {
   if (st) (void)hipStreamDestroy(st);
   if (a) (void)hipFree(a);
   if (b) (void)hipFree(b);
+  return rc;
+}
+
+// ---- objective and discrete adjoint (D3Q19 flow + solute, FP64) ----------
+
+static int adjoint_supported(const lbm_handle *h) {
+  const lbm_params &p = h->p;
+  if (h->Q != 19 || p.nsubs != 2) return lbm_fail("the adjoint needs D3Q19 flow + solute");
+  if (p.storage != LBM_STORE_F64 || p.compute != LBM_COMPUTE_F64) return lbm_fail("the adjoint runs in FP64");
+  if (p.halo_mode) return lbm_fail("the adjoint needs the periodic (halo_mode = 0) lattice");
+  if (p.sink != 0.0) return lbm_fail("the adjoint needs sink = 0");
+  if (p.flux_plane < 0 || p.flux_plane >= p.nk) return lbm_fail("the objective needs flux_plane");
+  return 0;
+}
+
+static size_t state_bytes(const lbm_handle *h) {
+  return (size_t)h->tiles * h->p.nsubs * h->Q * kTile * h->st_bytes;
+}
+
+extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int every, double *J) {
+  if (need_lattice(h) || adjoint_supported(h)) return 1;
+  if (nsteps < 1 || window < 1 || window > nsteps || every < 1 || !J) return lbm_fail("bad objective arguments");
+  const int t0 = h->time;
+  if (t0 + 1 < h->p.sigma_t_on || t0 + nsteps > h->p.sigma_t_off)
+    return lbm_fail("the solute must be active for the whole objective run");
+  free_objective(h);
+  const size_t sb = state_bytes(h);
+  HIP_OK(hipMalloc(&h->obj.jacc, sizeof(double)));
+  HIP_OK(hipMemsetAsync(h->obj.jacc, 0, sizeof(double), h->stream));
+  for (int q = 0; q < nsteps; q++) {
+    if (q % every == 0) {
+      void *c = nullptr;
+      HIP_OK(hipMalloc(&c, sb));
+      h->obj.ckpt.push_back(c);
+      HIP_OK(hipMemcpyAsync(c, h->F, sb, hipMemcpyDeviceToDevice, h->stream));
+    }
+    const bool in_window = q >= nsteps - window;
+    if (do_step(h, q == nsteps - 1, in_window ? h->obj.jacc : nullptr)) return 1;
+  }
+  double acc = 0.0;
+  HIP_OK(hipMemcpyAsync(&acc, h->obj.jacc, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  h->obj.t0 = t0;
+  h->obj.nsteps = nsteps;
+  h->obj.window = window;
+  h->obj.every = every;
+  h->obj.J = acc / ((double)window * (double)h->E);
+  h->obj.valid = 1;
+  *J = h->obj.J;
+  return 0;
+}
+
+extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
+  if (need_lattice(h) || adjoint_supported(h)) return 1;
+  if (!grad) return lbm_fail("null gradient output");
+  if (!h->obj.valid) return lbm_fail("lbm_objective_run has not been called");
+  const size_t sb = state_bytes(h);
+  const int K = h->obj.every, T = h->obj.nsteps, W_ = h->obj.window, t0 = h->obj.t0;
+  const int nseg = (int)h->obj.ckpt.size();
+  const int saved_time = h->time;
+  void *const saved_F = h->F;  // the replay points h->F at segment buffers
+  std::vector<void *> seg(K + 1, nullptr);
+  double *L = nullptr, *gacc = nullptr, *cgrad = nullptr;
+  const int nblk = 8 * h->nblk8;
+  int rc = 0;
+  auto fail = [&](const char *what, hipError_t e) {
+    rc = lbm_fail("adjoint: %s: %s", what, hipGetErrorString(e));
+    return rc;
+  };
+  hipError_t e;
+  for (int q = 0; q <= K && !rc; q++)
+    if ((e = hipMalloc(&seg[q], sb)) != hipSuccess) fail("segment buffers", e);
+  if (!rc && (e = hipMalloc(&L, sb)) != hipSuccess) fail("adjoint state", e);
+  if (!rc && (e = hipMalloc(&gacc, (size_t)nblk * 5 * sizeof(double))) != hipSuccess) fail("gradients", e);
+  if (!rc && (e = hipMalloc(&cgrad, (size_t)h->E * sizeof(double))) != hipSuccess) fail("inlet gradient", e);
+  if (!rc) {
+    (void)hipMemsetAsync(L, 0, sb, h->stream);
+    (void)hipMemsetAsync(gacc, 0, (size_t)nblk * 5 * sizeof(double), h->stream);
+    (void)hipMemsetAsync(cgrad, 0, (size_t)h->E * sizeof(double), h->stream);
+  }
+  const double seed = 1.0 / ((double)W_ * (double)h->E);
+  const bool off32 = sb <= 0xffffffffull;
+  for (int sgi = nseg - 1; sgi >= 0 && !rc; sgi--) {
+    const int ts = t0 + sgi * K;                 // state seg[0] is before step ts+1
+    const int len = std::min(K, t0 + T - ts);
+    if ((e = hipMemcpyAsync(seg[0], h->obj.ckpt[sgi], sb, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess) {
+      fail("checkpoint restore", e);
+      break;
+    }
+    // replay the segment out of place, keeping every state after its BCs
+    for (int q = 1; q <= len && !rc; q++) {
+      const int t = ts + q;
+      h->F = seg[q - 1];
+      if (launch_bcs(h, (t & 1) ? 1 : 2, t)) {
+        rc = 1;
+        break;
+      }
+      StepArgs<double, double> A = make_args<double, double>(h);
+      A.Fo = (double *)seg[q];
+      A.solute_on = 1;
+      dim3 grid(nblk), block(kBlock);
+      if (t & 1) {
+        if (off32) hipLaunchKernelGGL((k_half_step<19, 2, double, double, true, true>), grid, block, 0, h->stream, A);
+        else hipLaunchKernelGGL((k_half_step<19, 2, double, double, true, false>), grid, block, 0, h->stream, A);
+      } else {
+        if (off32) hipLaunchKernelGGL((k_half_step<19, 2, double, double, false, true>), grid, block, 0, h->stream, A);
+        else hipLaunchKernelGGL((k_half_step<19, 2, double, double, false, false>), grid, block, 0, h->stream, A);
+      }
+      if ((e = hipGetLastError()) != hipSuccess) fail("replay", e);
+    }
+    // reverse sweep over the segment
+    for (int q = len; q >= 1 && !rc; q--) {
+      const int t = ts + q;
+      const bool odd = t & 1;
+      AdjArgs J;
+      h->F = seg[q - 1];
+      J.A = make_args<double, double>(h);
+      J.L = L;
+      J.seed = t > t0 + T - W_ ? seed : 0.0;
+      J.zf = h->p.flux_plane;
+      J.gacc = gacc;
+      dim3 grid(nblk), block(kBlock);
+      if (odd) {
+        if (off32) hipLaunchKernelGGL((k_adjoint_step<true, true>), grid, block, 0, h->stream, J);
+        else hipLaunchKernelGGL((k_adjoint_step<true, false>), grid, block, 0, h->stream, J);
+      } else {
+        if (off32) hipLaunchKernelGGL((k_adjoint_step<false, true>), grid, block, 0, h->stream, J);
+        else hipLaunchKernelGGL((k_adjoint_step<false, false>), grid, block, 0, h->stream, J);
+      }
+      if ((e = hipGetLastError()) != hipSuccess) {
+        fail("adjoint step", e);
+        break;
+      }
+      const lbm_params &p = h->p;
+      if (p.zface_pressure || p.zface_solute) {
+        for (int top = 0; top < 2; top++) {
+          AdjBcArgs B;
+          B.A = J.A;
+          B.L = L;
+          B.list = top ? h->face_hi : h->face_lo;
+          B.n = top ? h->n_hi : h->n_lo;
+          B.phase = odd ? 1 : 2;
+          if (!B.n) continue;
+          ZfaceArgs Z;
+          Z.top = top;
+          Z.flow = p.zface_pressure;
+          Z.solute = p.zface_solute;
+          Z.rho_b = top ? p.rho_top : p.rho_bot;
+          Z.cmap = h->cmap;
+          Z.c_b = p.c_bot;
+          hipLaunchKernelGGL(k_adjoint_zface, dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B, Z, cgrad);
+          if ((e = hipGetLastError()) != hipSuccess) fail("adjoint BC", e);
+        }
+      }
+    }
+  }
+  h->F = saved_F;
+  if (!rc) {
+    std::vector<double> part((size_t)nblk * 5);
+    if ((e = hipMemcpyAsync(part.data(), gacc, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(h->stream)) != hipSuccess) {
+      fail("gradient read-back", e);
+    } else {
+      for (int q = 0; q < 5; q++) grad[q] = 0.0;
+      for (int b = 0; b < nblk; b++)
+        for (int q = 0; q < 5; q++) grad[q] += part[(size_t)b * 5 + q];
+      if (grad_cmap) {
+        if ((e = hipMemcpy(grad_cmap, cgrad, (size_t)h->E * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+          fail("inlet gradient read-back", e);
+      }
+    }
+  }
+  (void)hipStreamSynchronize(h->stream);
+  for (void *q : seg)
+    if (q) (void)hipFree(q);
+  if (L) (void)hipFree(L);
+  if (gacc) (void)hipFree(gacc);
+  if (cgrad) (void)hipFree(cgrad);
+  h->time = saved_time;
   return rc;
 }

@@ -92,7 +92,9 @@ __device__ __forceinline__ void st_stream(T *p, T v) {
 
 template <typename ST, typename CT>
 struct StepArgs {
-  ST *F;          // tiles [nf/64][NS][Q][64]
+  ST *F;          // tiles [nf/64][NS][Q][64]: populations the half-step reads
+  ST *Fo;         // where it writes them (== F in place; another state
+                  // buffer when the adjoint replays a segment)
   Geo G;
   int nf;         // fluid nodes
   int nblk8;      // launched blocks / 8
@@ -161,7 +163,7 @@ __device__ __forceinline__ void scatter(const StepArgs<ST, CT> &A, int idx, cons
     else
       off = foff<Q, NS>(SUBS, a, idx);
     st_stream<!ODD>(
-        elem<OFF32>(A.F, off), Codec<ST, CT>::enc(f[a], A.scale[SUBS], shift_of<Q, ST, CT, SUBS>(A, a)));
+        elem<OFF32>(A.Fo, off), Codec<ST, CT>::enc(f[a], A.scale[SUBS], shift_of<Q, ST, CT, SUBS>(A, a)));
   }
 }
 
