@@ -74,6 +74,10 @@ def lib():
         L.orc_reduce_sums.argtypes = [vp, dp]
         L.orc_last_flux.argtypes = [vp, dp, dp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_tlm_objective.restype = ctypes.c_int
+        L.orc_tlm_objective.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint8),
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        dp, dp, dp, dp, dp, dp]
         L.orc_num_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -251,3 +255,21 @@ def set_threads(n):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def tlm_objective(mask, rho_bot, rho_top, zf, nsteps, window, params, cmap, dparams=None, dcmap=None):
+    """(J, dJ) of the solute-flux objective from the FP64 tangent-linear
+    model (oracle/lbm_tlm.cpp); params = (gx, gy, gz, tau, tau_s)."""
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    nk, nj, ni = m.shape
+    p = np.ascontiguousarray(list(params) + [0.0], dtype=np.float64)
+    c = np.ascontiguousarray(cmap, dtype=np.float64).ravel()
+    J, dJ = ctypes.c_double(), ctypes.c_double()
+    dpp = np.ascontiguousarray(dparams, dtype=np.float64) if dparams is not None else None
+    dcc = np.ascontiguousarray(dcmap, dtype=np.float64).ravel() if dcmap is not None else None
+    rc = lib().orc_tlm_objective(ni, nj, nk, m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), rho_bot, rho_top,
+                                 zf, nsteps, window, _dp(p), _dp(c), _dp(dpp) if dpp is not None else None,
+                                 _dp(dcc) if dcc is not None else None, ctypes.byref(J), ctypes.byref(dJ))
+    if rc:
+        raise ValueError("orc_tlm_objective: bad arguments")
+    return J.value, dJ.value

@@ -1,0 +1,98 @@
+"""GPU discrete adjoint of the solute-flux objective (lbm_objective_run /
+lbm_adjoint_run) against the FP64 tangent-linear model (oracle/lbm_tlm.cpp)
+and against central finite differences of the GPU forward run."""
+import sys
+
+import numpy as np
+import pytest
+
+import _pkg
+from oracle.pyoracle import tlm_objective
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def setup(ni=10, nj=8, nk=16, seed=11):
+    L = _pkg.load()
+    mask, _ = L.spheres3d(ni, nj, nk, seed, 2.0, 3.5, 0.6, zbuf=3)
+    cmap = np.random.default_rng(4).uniform(0.8, 1.2, (nj, ni))
+    return L, mask, cmap
+
+
+def gpu_objective(L, mask, cmap, params, nsteps, window, every):
+    nk, nj, ni = mask.shape
+    lat = L.Lattice(mask=mask, dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(params[3], params[4]),
+                    gaccel=tuple(params[:3]) + (0.0, 0.0, 0.0), zface_pressure=1, rho_bot=1.001,
+                    rho_top=1.0, zface_solute=1, c_bot_map=cmap, flux_plane=nk - 2)
+    lat.init_rest()
+    J = lat.objective(nsteps, window, every)
+    return lat, J
+
+
+PARAMS = np.array([2e-6, -1e-6, 1e-5, 0.8, 0.75])
+
+
+def test_adjoint_matches_tangent_linear_model():
+    say("case adjoint vs TLM 10x8x16, 60 steps, window 20, checkpoints every 25")
+    L, mask, cmap = setup()
+    zf = mask.shape[0] - 2
+    lat, J = gpu_objective(L, mask, cmap, PARAMS, 60, 20, 25)
+    grad, gmap = lat.adjoint()
+    J_tlm, _ = tlm_objective(mask, 1.001, 1.0, zf, 60, 20, PARAMS, cmap)
+    say(f"J gpu {J:.15e} tlm {J_tlm:.15e}")
+    assert abs(J - J_tlm) <= 1e-12 * abs(J_tlm)
+    for q in range(5):
+        dp = np.zeros(5)
+        dp[q] = 1.0
+        _, dJ = tlm_objective(mask, 1.001, 1.0, zf, 60, 20, PARAMS, cmap, dp, np.zeros_like(cmap))
+        say(f"param {q}: adjoint {grad[q]:.12e} tlm {dJ:.12e}")
+        assert abs(grad[q] - dJ) <= 1e-9 * max(abs(dJ), 1e-30) + 1e-14
+    dc = np.random.default_rng(9).uniform(-1, 1, cmap.shape)
+    _, dJ = tlm_objective(mask, 1.001, 1.0, zf, 60, 20, PARAMS, cmap, np.zeros(5), dc)
+    adj = float(np.sum(gmap * dc))
+    say(f"inlet map direction: adjoint {adj:.12e} tlm {dJ:.12e}")
+    assert abs(adj - dJ) <= 1e-9 * abs(dJ)
+    # the map gradient lives on the inlet face's fluid nodes only
+    assert np.all(gmap[mask[0] == 1] == 0.0)
+
+
+def test_adjoint_matches_central_differences():
+    say("case adjoint vs central differences 12x10x20, 80 steps")
+    L, mask, cmap = setup(12, 10, 20, seed=5)
+    lat, J = gpu_objective(L, mask, cmap, PARAMS, 80, 30, 40)
+    grad, gmap = lat.adjoint()
+    for q, h in ((2, 1e-8), (3, 1e-6), (4, 1e-6)):
+        dp = np.zeros(5)
+        dp[q] = h
+        _, Jp = gpu_objective(L, mask, cmap, PARAMS + dp, 80, 30, 40)
+        _, Jm = gpu_objective(L, mask, cmap, PARAMS - dp, 80, 30, 40)
+        fd = (Jp - Jm) / (2 * h)
+        say(f"param {q}: adjoint {grad[q]:.10e} fd {fd:.10e}")
+        assert abs(grad[q] - fd) <= 1e-5 * abs(fd)
+    i, j = 3, 4
+    h = 1e-6
+    cp, cm = cmap.copy(), cmap.copy()
+    cp[j, i] += h
+    cm[j, i] -= h
+    _, Jp = gpu_objective(L, mask, cp, PARAMS, 80, 30, 40)
+    _, Jm = gpu_objective(L, mask, cm, PARAMS, 80, 30, 40)
+    fd = (Jp - Jm) / (2 * h)
+    say(f"inlet node ({i},{j}): adjoint {gmap[j, i]:.10e} fd {fd:.10e}")
+    assert abs(gmap[j, i] - fd) <= 1e-5 * abs(fd)
+
+
+def test_adjoint_rejects_unsupported_configurations():
+    L, mask, cmap = setup()
+    nk, nj, ni = mask.shape
+    lat = L.Lattice(mask=mask, dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(0.8, 0.75), storage=1,
+                    flux_plane=nk - 2, zface_solute=1)
+    with pytest.raises(L.LbmError):
+        lat.objective(10, 5)
+    lat = L.Lattice(mask=mask, dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(0.8, 0.75), flux_plane=nk - 2)
+    with pytest.raises(L.LbmError):
+        lat.adjoint()

@@ -1,0 +1,63 @@
+"""The FP64 tangent-linear model (oracle/lbm_tlm.cpp) that checks the GPU
+adjoint: its objective equals the AA oracle's on the same lattice, and its
+directional derivatives equal central finite differences."""
+import numpy as np
+import pytest
+
+import _pkg
+from oracle.pyoracle import Oracle, tlm_objective
+
+
+def case():
+    L = _pkg.load()
+    ni, nj, nk = 10, 8, 16
+    mask, _ = L.spheres3d(ni, nj, nk, 11, 2.0, 3.5, 0.6, zbuf=3)
+    rng = np.random.default_rng(4)
+    cmap = rng.uniform(0.8, 1.2, (nj, ni))
+    return mask, cmap
+
+
+def oracle_objective(mask, cmap, params, nsteps, window, zf):
+    nk, nj, ni = mask.shape
+    o = Oracle(dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(params[3], params[4]),
+               gaccel=tuple(params[:3]) + (0.0, 0.0, 0.0), zface_pressure=1, rho_bot=1.001, rho_top=1.0,
+               zface_solute=1, c_bot_map=cmap, flux_plane=zf)
+    o.set_solids(mask)
+    o.init_rest()
+    acc = 0.0
+    for t in range(1, nsteps + 1):
+        o.step(False)
+        if t > nsteps - window:
+            acc += o.last_flux()[0]
+    return acc / (window * ni * nj)
+
+
+def test_tlm_objective_equals_oracle():
+    mask, cmap = case()
+    params = (0.0, 0.0, 1e-5, 0.8, 0.75)
+    zf = mask.shape[0] - 2
+    J_tlm, _ = tlm_objective(mask, 1.001, 1.0, zf, 60, 20, params, cmap)
+    J_orc = oracle_objective(mask, cmap, params, 60, 20, zf)
+    assert J_orc != 0.0
+    assert abs(J_tlm - J_orc) <= 1e-12 * abs(J_orc)
+
+
+@pytest.mark.parametrize("which", range(6))
+def test_tlm_derivative_equals_central_difference(which):
+    mask, cmap = case()
+    params = np.array([0.0, 0.0, 1e-5, 0.8, 0.75])
+    zf = mask.shape[0] - 2
+    rng = np.random.default_rng(7)
+    dp = np.zeros(5)
+    dc = np.zeros_like(cmap)
+    if which < 5:
+        dp[which] = 1.0
+        h = 1e-6 if which >= 3 else 1e-6
+    else:
+        dc = rng.uniform(-1, 1, cmap.shape)
+        h = 1e-5
+    _, dJ = tlm_objective(mask, 1.001, 1.0, zf, 50, 15, params, cmap, dp, dc)
+    Jp, _ = tlm_objective(mask, 1.001, 1.0, zf, 50, 15, params + h * dp, cmap + h * dc)
+    Jm, _ = tlm_objective(mask, 1.001, 1.0, zf, 50, 15, params - h * dp, cmap - h * dc)
+    fd = (Jp - Jm) / (2 * h)
+    assert abs(dJ - fd) <= 1e-6 * abs(fd) + 1e-12, (dJ, fd)
