@@ -70,6 +70,144 @@ def run_workload(L, mask, kw, steps, warmup, label):
     return info, wall, prof, sums
 
 
+def adjoint_timing(L, n=128, steps=2000, window=200, every=100):
+    """discrete-adjoint workload: D3Q19 flow + solute 128^3, spheres to
+    porosity 0.40 (seed 404), inlet map U[0.8, 1.2] (seed 4040); time of the
+    objective (forward) run and of the adjoint sweep"""
+    mask, phi = L.spheres3d(n, n, n, 404, 6.0, 14.0, 0.40, zbuf=4)
+    cmap = np.random.default_rng(4040).uniform(0.8, 1.2, (n, n))
+    lat = L.Lattice(mask=mask, dims=3, ni=n, nj=n, nk=n, nsubs=2, tau=(0.8, 0.75),
+                    gaccel=(0.0, 0.0, 1e-6, 0, 0, 0), zface_pressure=1, rho_bot=1.001, rho_top=1.0,
+                    zface_solute=1, c_bot_map=cmap, flux_plane=n - 2)
+    lat.init_rest()
+    lat.run(2, write_last=False)
+    lat.init_rest()
+    lat.time = 0
+    t0 = time.perf_counter()
+    J = lat.objective(steps, window, every)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    grad, gmap = lat.adjoint()
+    t_adj = time.perf_counter() - t0
+    lat.close()
+    log(f"[adjoint] J {J:.6e} forward {t_fwd:.3f}s adjoint {t_adj:.3f}s grad {grad}")
+    return {"lattice": [n, n, n], "porosity": round(phi, 5), "steps": steps, "window": window,
+            "checkpoint_every": every, "J": J, "dJ_dgz": grad[2], "dJ_dtau": grad[3], "dJ_dtau_s": grad[4],
+            "dJ_dcmap_sum": float(gmap.sum()), "forward_s": round(t_fwd, 4), "adjoint_s": round(t_adj, 4),
+            "adjoint_over_forward": round(t_adj / t_fwd, 3),
+            "gradient_over_forward": round((t_fwd + t_adj) / t_fwd, 3)}
+
+
+def recorded_traffic(kernel_sig):
+    """HBM bytes per launch of a kernel as recorded by the two --pmc runs
+    summarised in profiles/hbm_traffic_d3q19_flow_fp64_256.md (FETCH_SIZE and
+    WRITE_SIZE scaled by the same counters on a plain copy kernel)"""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_d3q19_flow_fp64_256.md")
+    try:
+        for line in open(path):
+            if kernel_sig in line and line.startswith("|"):
+                return float(line.strip().strip("|").split("|")[-1]) * 1e9, os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
+def permeability(sums, n_nodes, tau, rho_in, rho_out, length):
+    """Darcy permeability of a pressure-driven run from LBM_REDUCE_SUMS:
+    k = nu <u_z> / (dp / L), nu = (tau - 1/2)/3, dp = (rho_in - rho_out)/3,
+    <u_z> the superficial average (solid nodes count as zero velocity)"""
+    nu = (tau - 0.5) / 3.0
+    return nu * (sums[3] / n_nodes) / ((rho_in - rho_out) / 3.0 / length)
+
+
+def precision_table(L, n=192, steps=4000):
+    """configuration 3: D3Q19 flow + solute 192^3, porosity 0.45 (seed 303),
+    solute c = 1 at the inlet, zero gradient at the outlet, 4000 steps in
+    each storage precision; errors against the FP64 run, which the tests
+    show equal to the FP64 CPU oracle bit for bit"""
+    mask, phi = L.spheres3d(n, n, n, 303, 6.0, 14.0, 0.45, zbuf=4)
+    base = dict(dims=3, ni=n, nj=n, nk=n, nsubs=2, tau=(0.8, 0.75), zface_pressure=1, rho_bot=1.001,
+                rho_top=1.0, zface_solute=1, c_bot=1.0, flux_plane=n - 2)
+    fl = mask.reshape(-1) != 1
+    runs = {}
+    for name, st, ct in (("fp64", 0, 0), ("fp64_store_fp32_collision", 0, 1), ("fp32_shifted", 1, 1),
+                         ("fp32_shifted_fp64_collision", 1, 0), ("fp16_scaled", 2, 1)):
+        lat = L.Lattice(mask=mask, storage=st, compute=ct, store_shift_c=0.5 if st == 2 else 0.0, **base)
+        lat.init_rest()
+        t0 = time.perf_counter()
+        lat.run(steps, write_last=True)
+        wall = time.perf_counter() - t0
+        rho, ux, uy, uz = lat.mv(0)
+        c = lat.mv(1)[0]
+        flux = lat.last_flux()[0]
+        k = permeability(lat.reduce_sums(), n ** 3, 0.8, 1.001, 1.0, n - 1)
+        runs[name] = (rho[fl] - 1.0, np.stack([ux[fl], uy[fl], uz[fl]]), c[fl], flux, k, wall)
+        lat.close()
+        log(f"[precision] {name} {wall:.2f}s flux {flux:.6e} k {k:.6e}")
+    ref = runs["fp64"]
+    out = {"lattice": [n, n, n], "porosity": round(phi, 5), "steps": steps,
+           "permeability_fp64": ref[4], "outlet_flux_fp64": ref[3]}
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+    for name, r in runs.items():
+        if name == "fp64":
+            continue
+        out[name] = {"drho": rel(r[0], ref[0]), "u": rel(r[1], ref[1]), "c": rel(r[2], ref[2]),
+                     "outlet_flux": abs(r[3] - ref[3]) / abs(ref[3]), "permeability": abs(r[4] - ref[4]) / abs(ref[4]),
+                     "seconds": round(r[5], 3)}
+    return out
+
+
+def slab_c5(L, n=448, steps=2000):
+    """configuration 5: one GPU's z-slab, D3Q19 flow + solute 448^3,
+    FP32-shifted storage, porosity 0.35 (r 8-20, seed 505), pressure driven,
+    halo planes through the in-device periodic exchange; rate and drift
+    from the FP64 run of the same slab"""
+    mask, phi = L.spheres3d(n, n, n, 505, 8.0, 20.0, 0.35, zbuf=4)
+    base = dict(dims=3, ni=n, nj=n, nk=n, nsubs=2, tau=(0.8, 0.75), zface_pressure=1, rho_bot=1.001,
+                rho_top=1.0, zface_solute=1, c_bot=1.0, halo_mode=1, flux_plane=n - 2)
+    fl = mask.reshape(-1) != 1
+    res = {}
+    for name, st, ct, s in (("fp32_shifted", 1, 0, 4), ("fp64", 0, 0, 8)):
+        lat = L.Lattice(mask=mask, storage=st, compute=ct, **base)
+        info = lat.info()
+        lat.init_rest()
+        lat.profile(True)
+        lat.profile_read()
+        t0 = time.perf_counter()
+        lat.run(steps, write_last=True)
+        wall = time.perf_counter() - t0
+        prof = lat.profile_read()
+        B = algorithmic_bytes(info["fluid"], info["nodes"], 19, 2, s)
+        o_ms = prof["odd"][0] / max(prof["odd"][1], 1)
+        e_ms = prof["even"][0] / max(prof["even"][1], 1)
+        rho, ux, uy, uz = lat.mv(0)
+        c = lat.mv(1)[0]
+        k = permeability(lat.reduce_sums(), n ** 3, 0.8, 1.001, 1.0, n - 1)
+        res[name] = {"mlups": round(info["nodes"] * steps / wall / 1e6, 1),
+                     "mflups": round(info["fluid"] * steps / wall / 1e6, 1),
+                     "ms_per_step": round(wall / steps * 1e3, 4),
+                     "odd_frac": round(B / (o_ms * 1e-3) / HBM_PEAK, 4),
+                     "even_frac": round(B / (e_ms * 1e-3) / HBM_PEAK, 4),
+                     "halo_ms_per_step": round(prof["halo"][0] / steps, 5),
+                     "permeability": k, "outlet_flux": lat.last_flux()[0],
+                     "_fields": (rho[fl] - 1.0, np.stack([ux[fl], uy[fl], uz[fl]]), c[fl])}
+        lat.close()
+        log(f"[slab] {name} {wall:.2f}s {res[name]['mflups']} MFLUPS")
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+    a, b = res["fp32_shifted"].pop("_fields"), res["fp64"].pop("_fields")
+    res["drift_fp32_vs_fp64"] = {"drho": rel(a[0], b[0]), "u": rel(a[1], b[1]), "c": rel(a[2], b[2]),
+                                 "permeability": abs(res["fp32_shifted"]["permeability"] - res["fp64"]["permeability"])
+                                 / abs(res["fp64"]["permeability"])}
+    res["lattice"] = [n, n, n]
+    res["porosity"] = round(phi, 5)
+    res["steps"] = steps
+    return res
+
+
 def cpu_baseline(mask, kw, budget_s):
     sys.path.insert(0, ROOT)
     from oracle import pyoracle
@@ -123,6 +261,7 @@ def main():
     dom_ms = max(odd_ms, even_ms)
     achieved = B / (dom_ms * 1e-3)
     step_bw = B * args.steps / wall
+    traffic, traffic_src = recorded_traffic(f"k_half_step<19, 1, double, double, {'true' if dom == 'odd' else 'false'}, true>")
     result = {
         "metric": METRIC,
         "value": round(mlups, 2),
@@ -145,7 +284,7 @@ def main():
             "bound": "hbm", "kernel": f"k_half_step<19,1,double,double,{dom == 'odd'}>".replace("True", "true")
             .replace("False", "false"),
             "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic, "traffic_source": traffic_src,
             "bytes_per_launch": B, "avg_launch_ms": round(dom_ms, 5),
         },
         "extra": {
@@ -184,6 +323,9 @@ def main():
             ext["fp32_over_fp64_speed"] = round(ext["flow_solute_fp32_shifted"]["mflups"]
                                                 / ext["flow_solute_fp64"]["mflups"], 3)
         result["extra"].update(ext)
+        result["extra"]["adjoint_c4"] = adjoint_timing(L)
+        result["extra"]["precision_c3"] = precision_table(L)
+        result["extra"]["slab_c5"] = slab_c5(L)
 
     if not args.no_cpu:
         n_cpu, el, threads = cpu_baseline(mask, dict(base, nsubs=1), args.cpu_seconds)
