@@ -61,8 +61,8 @@ typedef struct lbm_params {
   int zface_solute;    /* 3-D solute: Dirichlet c at z=0, zero gradient at z=nk-1 */
   double c_bot;
   const double *c_bot_map; /* optional [nj][ni] inlet concentration */
-  int flux_plane;      /* plane (last axis) whose solute moments are summed
-                          every step; -1 = none */
+  int flux_plane;      /* plane (last axis) whose solute moments are summed on
+                          steps with a macroscopic write; -1 = none */
   int storage;         /* enum lbm_storage */
   int compute;         /* enum lbm_compute */
   double store_scale[2]; /* FP16: stored = (value - shift) * scale; 0 = default */
@@ -141,7 +141,8 @@ int lbm_download_macro(lbm_handle *h, int subs, double *rho, double *ux, double 
 
 /* FP64 reductions on the GPU over the last macroscopic write */
 int lbm_reduce(lbm_handle *h, int kind, int subs, double *out);
-/* solute moments of plane flux_plane in the last step */
+/* solute moments of plane flux_plane on the last step with a macroscopic
+ * write (or the last step of an objective run) */
 int lbm_last_flux(lbm_handle *h, double *flux, double *mass);
 
 /* geometry numbers: N, fluid nodes, population slots per direction

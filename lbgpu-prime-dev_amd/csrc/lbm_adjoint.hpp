@@ -62,12 +62,12 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_step(AdjArgs J) {
       for (int a = 1; a < Q; a++) nb[a] = r[a - 1];
     }
     double f[Q], g[Q], lam[Q], mu[Q];
-    gather<Q, 2, double, double, 0, ODD, OFF32>(A, idx, nb, f);
-    gather<Q, 2, double, double, 1, ODD, OFF32>(A, idx, nb, g);
+    gather2<Q, double, double, ODD, OFF32>(A, idx, nb, f, g);
 #pragma unroll
     for (int a = 0; a < Q; a++) {
-      lam[a] = J.L[push_off<ODD>(nb, 0, a, idx)];
-      mu[a] = J.L[push_off<ODD>(nb, 1, a, idx)];
+      const pair_t<double> v = *(const pair_t<double> *)elem<OFF32>(J.L, push_off<ODD>(nb, 0, a, idx));
+      lam[a] = v.x;
+      mu[a] = v.y;
     }
     const double rhoA = A.rhoA, tau = A.tau, om = A.inv_tau, oms = A.inv_tau_s;
     double drho = 0.0, u[3] = {0.0, 0.0, 0.0}, c = 0.0;
@@ -112,8 +112,10 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_step(AdjArgs J) {
     for (int b = 0; b < Q; b++) {
       const double lb_ = (1.0 - om) * lam[b] + om * Lr + (CX(b) * G[0] + CY(b) * G[1] + CZ(b) * G[2]) / rhoA;
       const double mb = (1.0 - oms) * mu[b] + oms * Sg + seed * CZ(b);
-      J.L[pull_off<ODD>(nb, 0, b, idx)] = lb_;
-      J.L[pull_off<ODD>(nb, 1, b, idx)] = mb;
+      pair_t<double> v;
+      v.x = lb_;
+      v.y = mb;
+      *(pair_t<double> *)elem<OFF32>(J.L, pull_off<ODD>(nb, 0, b, idx)) = v;
     }
 #pragma unroll
     for (int d = 0; d < 3; d++) acc[d] = tau * G[d];
@@ -234,11 +236,6 @@ __global__ void k_adjoint_zface(AdjBcArgs B, ZfaceArgs Z, double *cgrad) {
     if (Z.solute) adj_face<false>(B.L, nb, 1, idx, Z.top, false, dcb);
   }
   if (!Z.top && Z.solute && cgrad) cgrad[B.A.G.node[idx]] += dcb;
-}
-
-// running sum of the plane flux over the objective window
-__global__ void k_accumulate(const double *v, double *acc) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) acc[0] += v[0];
 }
 
 }  // namespace lbm

@@ -73,7 +73,7 @@ struct lbm_handle {
   void *halo_pos = nullptr, *halo_neg = nullptr;
   double *cmap = nullptr;
   double *red = nullptr;      // reduction partials
-  double *flux_dev = nullptr; // [2]
+  double *flux_dev = nullptr; // [2 + 2 * kFluxBlocks]: result, then block partials
   double flux = 0., fmass = 0.;
   int flux_valid = 0;
   int time = 0;
@@ -370,7 +370,7 @@ int halo_pack_unpack(lbm_handle *h, int phase, int pack, void *pos, void *neg) {
   });
 }
 
-int launch_flux(lbm_handle *h, int phase) {
+int launch_flux(lbm_handle *h, int phase, double *acc) {
   return dispatch(h, [&]<int Q, int NS, typename ST, typename CT>() -> int {
     if constexpr (NS == 2) {
       BcArgs<ST, CT> B;
@@ -379,7 +379,9 @@ int launch_flux(lbm_handle *h, int phase) {
       B.list = h->face_flux;
       B.n = h->n_flux;
       Prof pr(h, 4);
-      hipLaunchKernelGGL((k_plane_flux<Q, NS, ST, CT>), dim3(1), dim3(256), 0, h->stream, B, h->flux_dev);
+      hipLaunchKernelGGL((k_plane_flux<Q, NS, ST, CT>), dim3(kFluxBlocks), dim3(256), 0, h->stream, B,
+                         h->flux_dev + 2);
+      hipLaunchKernelGGL(k_plane_flux_final, dim3(1), dim3(64), 0, h->stream, h->flux_dev + 2, h->flux_dev, acc);
       HIP_OK(hipGetLastError());
     }
     return 0;
@@ -396,13 +398,10 @@ int do_step(lbm_handle *h, int write_mv, double *flux_acc = nullptr) {
   }
   if (launch_bcs(h, phase, t)) return 1;
   int sol_on = h->p.nsubs == 2 && t >= h->p.sigma_t_on && t <= h->p.sigma_t_off;
-  if (h->n_flux && sol_on) {
-    if (launch_flux(h, phase)) return 1;
+  // the plane flux is taken on steps with a macroscopic write and inside an objective window
+  if (h->n_flux && sol_on && (write_mv || flux_acc)) {
+    if (launch_flux(h, phase, flux_acc)) return 1;
     h->flux_valid = 1;
-    if (flux_acc) {
-      hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(64), 0, h->stream, h->flux_dev, flux_acc);
-      HIP_OK(hipGetLastError());
-    }
   }
   if (launch_half(h, odd, t, write_mv)) return 1;
   h->time = t;
@@ -462,7 +461,7 @@ extern "C" int lbm_create(const lbm_params *p, lbm_handle **out) {
     }
   }
   if (hipMalloc(&h->red, sizeof(double) * 12 * kRedBlocks) != hipSuccess ||
-      hipMalloc(&h->flux_dev, sizeof(double) * 2) != hipSuccess) {
+      hipMalloc(&h->flux_dev, sizeof(double) * (2 + 2 * kFluxBlocks)) != hipSuccess) {
     lbm_destroy(h);
     return lbm_fail("hipMalloc failed");
   }

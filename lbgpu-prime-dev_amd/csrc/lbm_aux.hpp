@@ -165,12 +165,17 @@ __global__ __launch_bounds__(256) void k_reduce_mv(const double *mv, int nf, dou
 }
 
 // solute moments of one plane: sum_a c_a,last g_a and sum_a g_a on the
-// populations the coming half-step collides (one block, fixed order)
+// populations the coming half-step collides.  Stage 1: kFluxBlocks blocks
+// stride over the plane's nodes and leave per-block partial sums; stage 2:
+// one block adds them in a fixed order (deterministic) and optionally adds
+// the flux to a running sum.
+constexpr int kFluxBlocks = 64;
+
 template <int Q, int NS, typename ST, typename CT>
-__global__ __launch_bounds__(256) void k_plane_flux(BcArgs<ST, CT> B, double *out) {
+__global__ __launch_bounds__(256) void k_plane_flux(BcArgs<ST, CT> B, double *partial) {
   double v[2] = {0.0, 0.0};
   const int kind[2] = {0, 0};
-  for (int t = threadIdx.x; t < B.n; t += blockDim.x) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < B.n; t += gridDim.x * blockDim.x) {
     const int idx = B.list[t];
     int nb[Q];
     CT f[Q];
@@ -184,7 +189,20 @@ __global__ __launch_bounds__(256) void k_plane_flux(BcArgs<ST, CT> B, double *ou
     v[0] += jl;
     v[1] += m;
   }
-  block_reduce<2>(v, kind, out);
+  block_reduce<2>(v, kind, partial + 2 * blockIdx.x);
+}
+
+__global__ __launch_bounds__(64) void k_plane_flux_final(const double *partial, double *out, double *acc) {
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int q = 0; q < kFluxBlocks; q++) {
+      a += partial[2 * q];
+      b += partial[2 * q + 1];
+    }
+    out[0] = a;
+    out[1] = b;
+    if (acc) acc[0] += a;
+  }
 }
 
 // plain copy of n doubles: the bandwidth (and HBM byte counter) reference

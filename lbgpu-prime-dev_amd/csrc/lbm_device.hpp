@@ -1,7 +1,8 @@
 // Device code of the stream-collide library (gfx950, wave64).
 //
 // Storage: populations of fluid nodes only, in tiles of 64 consecutive
-// compact nodes (lbm_index.hpp), compact order = dense order (i fastest).
+// compact nodes (lbm_index.hpp), compact order = dense order (i fastest);
+// with a solute, the two populations of a slot are adjacent.
 // Slot meaning follows the reference's in-place AA scheme (Bailey et al.,
 // ICPP 2009; src/k_stream_collide_stream.c, src/k_collide.c): before an odd
 // half-step slot a at x holds the post-collision f_a(x); the odd half-step
@@ -167,6 +168,48 @@ __device__ __forceinline__ void scatter(const StepArgs<ST, CT> &A, int idx, cons
   }
 }
 
+// Flow and solute together: one 2-wide access per slot moves both
+// populations (the pair layout of foff<Q, 2>).
+template <typename ST>
+using pair_t = ST __attribute__((ext_vector_type(2)));
+
+template <int Q, typename ST, typename CT, bool ODD, bool OFF32>
+__device__ __forceinline__ void gather2(const StepArgs<ST, CT> &A, int idx, const int (&nb)[Q], CT (&f)[Q],
+                                        CT (&g)[Q]) {
+#pragma unroll
+  for (int a = 0; a < Q; a++) {
+    uint32_t off;
+    if (a == 0)
+      off = foff<Q, 2>(0, 0, idx);
+    else if (ODD)
+      off = nb[a] >= 0 ? foff<Q, 2>(0, a, nb[a]) : foff<Q, 2>(0, OPP(a), idx);
+    else
+      off = foff<Q, 2>(0, OPP(a), idx);
+    const pair_t<ST> v = ld_stream<!ODD>((const pair_t<ST> *)elem<OFF32>(A.F, off));
+    f[a] = Codec<ST, CT>::dec(v.x, A.inv_scale[0], (CT)0);
+    g[a] = Codec<ST, CT>::dec(v.y, A.inv_scale[1], shift_of<Q, ST, CT, 1>(A, a));
+  }
+}
+
+template <int Q, typename ST, typename CT, bool ODD, bool OFF32>
+__device__ __forceinline__ void scatter2(const StepArgs<ST, CT> &A, int idx, const int (&nb)[Q],
+                                         const CT (&f)[Q], const CT (&g)[Q]) {
+#pragma unroll
+  for (int a = 0; a < Q; a++) {
+    uint32_t off;
+    if (a == 0)
+      off = foff<Q, 2>(0, 0, idx);
+    else if (ODD)
+      off = nb[OPP(a)] >= 0 ? foff<Q, 2>(0, OPP(a), nb[OPP(a)]) : foff<Q, 2>(0, a, idx);
+    else
+      off = foff<Q, 2>(0, a, idx);
+    pair_t<ST> v;
+    v.x = Codec<ST, CT>::enc(f[a], A.scale[0], (CT)0);
+    v.y = Codec<ST, CT>::enc(g[a], A.scale[1], shift_of<Q, ST, CT, 1>(A, a));
+    st_stream<!ODD>((pair_t<ST> *)elem<OFF32>(A.Fo, off), v);
+  }
+}
+
 // Moments, force, macroscopic write and BGK relaxation of one subs, in the
 // order of src/k_stream_collide_stream.c:80-390 and
 // src/lattice.c:2850-2944.  SUBS 0: He-Luo incompressible flow on shifted
@@ -234,14 +277,18 @@ __global__ __launch_bounds__(kBlock) void k_half_step(StepArgs<ST, CT> A) {
   nb[0] = idx;
   if (ODD) neighbors<Q>(A.G, idx, nb);
   CT f[Q], u[3];
+  if (NS == 2 && A.solute_on) {
+    CT g[Q];
+    gather2<Q, ST, CT, ODD, OFF32>(A, idx, nb, f, g);
+    collide<Q, ST, CT, 0, ODD>(A, idx, f, u);
+    collide<Q, ST, CT, 1, ODD>(A, idx, g, u);
+    scatter2<Q, ST, CT, ODD, OFF32>(A, idx, nb, f, g);
+    return;
+  }
+  // flow alone (the solute, when present, is outside its active time window)
   gather<Q, NS, ST, CT, 0, ODD, OFF32>(A, idx, nb, f);
   collide<Q, ST, CT, 0, ODD>(A, idx, f, u);
   scatter<Q, NS, ST, CT, 0, ODD, OFF32>(A, idx, nb, f);
-  if (NS == 2 && A.solute_on) {
-    gather<Q, NS, ST, CT, 1, ODD, OFF32>(A, idx, nb, f);
-    collide<Q, ST, CT, 1, ODD>(A, idx, f, u);
-    scatter<Q, NS, ST, CT, 1, ODD, OFF32>(A, idx, nb, f);
-  }
 }
 
 // ---- boundary conditions ------------------------------------------------

@@ -2,9 +2,13 @@
 // CPU index checker (tests/index_check.cpp compiles this header with g++).
 //
 // Populations live in tiles of kTile consecutive fluid nodes:
-//   element (subs s, slot a, fluid node x) at
-//   (x / kTile) * NS*Q*kTile + (s*Q + a) * kTile + x % kTile
-// so that the Q*NS rows of one wave's 64 nodes form one contiguous block.
+//   one distribution:  element (slot a, node x) at
+//     (x / kTile) * Q*kTile + a * kTile + x % kTile
+//   flow + solute:     element (subs s, slot a, node x) at
+//     (x / kTile) * 2*Q*kTile + (a * kTile + x % kTile) * 2 + s
+// so that the rows of one wave's 64 nodes form one contiguous block, and
+// the flow and solute populations of a slot sit side by side (one 2-wide
+// load or store moves both).
 //
 // Neighbours are not stored per node.  Each lattice row (fixed j, k) is
 // split in 64-node words; for word w of row r, RowWord holds the fluid
@@ -80,7 +84,8 @@ struct Geo {
 
 template <int Q, int NS>
 LBM_HD uint32_t foff(int s, int a, uint32_t x) {
-  return (x / kTile) * (uint32_t)(NS * Q * kTile) + (uint32_t)((s * Q + a) * kTile) + (x % kTile);
+  if (NS == 2) return (x / kTile) * (uint32_t)(2 * Q * kTile) + (uint32_t)(a * kTile + (x % kTile)) * 2u + (uint32_t)s;
+  return (x / kTile) * (uint32_t)(Q * kTile) + (uint32_t)(a * kTile) + (x % kTile);
 }
 
 // fluid test and compact index of node (i, row r): one 16-byte load and

@@ -145,8 +145,8 @@ def test_3d_zface_pressure_solute_flux(halo):
     o.init_rest()
     g.init_rest()
     for t in range(1, 61):
-        o.step(t % 10 == 0)
-        g.step(t % 10 == 0)
+        o.step(True)
+        g.step(True)
         fo, fg = o.last_flux(), g.last_flux()
         assert abs(fg[0] - fo[0]) <= 1e-12 * max(abs(fo[0]), 1e-300) + 1e-300, (t, fo, fg)
         assert abs(fg[1] - fo[1]) <= 1e-12 * abs(fo[1]) + 1e-300
