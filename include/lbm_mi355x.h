@@ -31,6 +31,8 @@ typedef struct lbm_handle lbm_handle;
 
 enum lbm_storage { LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16 = 2 };
 enum lbm_compute { LBM_COMPUTE_F64 = 0, LBM_COMPUTE_F32 = 1 };
+/* what a step writes to the macroscopic buffer */
+enum lbm_macro { LBM_MACRO_NONE = 0, LBM_MACRO_FRAME = 1, LBM_MACRO_PEST = 2 };
 
 /* Parameters: the fields read_params() fills for the hot path
  * (src/params.h:40-661, src/lattice.h:113-300) plus storage precision and
@@ -107,8 +109,12 @@ int lbm_get_time(const lbm_handle *h, int *time);
 
 /* One LBM step at time+1: odd times run the halo exchange (halo_mode),
  * bcs_1 and k_stream_collide_stream (src/lbgpu_prime.cu:382-530); even
- * times the exchange, bcs_2 and k_collide (:612-752).  write_macro stores
- * the pre-collision moments (is_end_of_frame_mem_c). */
+ * times the exchange, bcs_2 and k_collide (:612-752).  write_macro is an
+ * lbm_macro value: LBM_MACRO_FRAME stores the pre-collision moments
+ * (is_end_of_frame_mem_c, src/k_stream_collide_stream.c:262-304);
+ * LBM_MACRO_PEST stores only the flow velocity and the solute
+ * concentration (pest_output_flag_c, :305-334); other nonzero values act as
+ * LBM_MACRO_FRAME. */
 int lbm_step(lbm_handle *h, int write_macro);
 /* nsteps steps, macroscopic write on the last one if write_macro_last */
 int lbm_run(lbm_handle *h, int nsteps, int write_macro_last);
@@ -138,6 +144,11 @@ int lbm_halo_periodic(lbm_handle *h, int phase);
 /* mv D2H (src/lbgpu_prime.cu:534-583): rho (or c for subs 1) and u_eq per
  * node; solid nodes get 0; uz may be NULL in 2-D */
 int lbm_download_macro(lbm_handle *h, int subs, double *rho, double *ux, double *uy, double *uz);
+/* mv H2D (src/lbgpu_prime.cu:345-352): sets the macroscopic buffer at the
+ * fluid nodes from [N] host arrays; a NULL component is left as it is.
+ * Steps with LBM_MACRO_PEST overwrite only part of the buffer. */
+int lbm_upload_macro(lbm_handle *h, int subs, const double *rho, const double *ux, const double *uy,
+                     const double *uz);
 
 /* FP64 reductions on the GPU over the last macroscopic write */
 int lbm_reduce(lbm_handle *h, int kind, int subs, double *out);

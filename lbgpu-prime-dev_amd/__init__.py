@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "liblbm_mi355x.so")
 STORE_F64, STORE_F32, STORE_F16 = 0, 1, 2
 COMPUTE_F64, COMPUTE_F32 = 0, 1
 REDUCE_REF, REDUCE_SUMS = 0, 1
+MACRO_NONE, MACRO_FRAME, MACRO_PEST = 0, 1, 2
 PROFILE_CLASSES = ("odd", "even", "bc", "halo", "flux")
 
 
@@ -86,6 +87,7 @@ def lib():
             "lbm_halo_unpack": [vp, ip, vp, vp],
             "lbm_halo_periodic": [vp, ip],
             "lbm_download_macro": [vp, ip, dp, dp, dp, dp],
+            "lbm_upload_macro": [vp, ip, dp, dp, dp, dp],
             "lbm_reduce": [vp, ip, ip, dp],
             "lbm_last_flux": [vp, dp, dp],
             "lbm_info": [vp, lp, lp, lp],
@@ -241,17 +243,17 @@ class Lattice:
         _check(lib().lbm_set_time(self.h, int(t)))
 
     def step(self, write_mv=False):
-        _check(lib().lbm_step(self.h, int(bool(write_mv))))
+        _check(lib().lbm_step(self.h, int(write_mv)))
 
     def run(self, n, write_last=True):
-        _check(lib().lbm_run(self.h, int(n), int(bool(write_last))))
+        _check(lib().lbm_run(self.h, int(n), int(write_last)))
 
     # the reference's pieces of one step pair
     def stream_collide_stream(self, time, write_mv=False):
-        _check(lib().lbm_step_odd(self.h, int(time), int(bool(write_mv))))
+        _check(lib().lbm_step_odd(self.h, int(time), int(write_mv)))
 
     def collide(self, time, write_mv=False):
-        _check(lib().lbm_step_even(self.h, int(time), int(bool(write_mv))))
+        _check(lib().lbm_step_even(self.h, int(time), int(write_mv)))
 
     def bcs_1(self, time):
         _check(lib().lbm_apply_bcs(self.h, 1, int(time)))
@@ -277,6 +279,15 @@ class Lattice:
         rho, ux, uy, uz = (np.empty(self.N) for _ in range(4))
         _check(lib().lbm_download_macro(self.h, subs, _dp(rho), _dp(ux), _dp(uy), _dp(uz)))
         return rho, ux, uy, uz
+
+    def set_mv(self, subs, rho=None, ux=None, uy=None, uz=None):
+        """lbm_upload_macro; None leaves a component as it is"""
+        arr = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (rho, ux, uy, uz)]
+        for a in arr:
+            if a is not None and a.size != self.N:
+                raise ValueError("macroscopic arrays hold one value per lattice node")
+        ptr = [None if a is None else _dp(a) for a in arr]
+        _check(lib().lbm_upload_macro(self.h, subs, *ptr))
 
     def reduce_ref(self, subs=0):
         out = np.empty(12)

@@ -658,7 +658,7 @@ extern "C" int lbm_step(lbm_handle *h, int write_macro) {
 extern "C" int lbm_run(lbm_handle *h, int nsteps, int write_macro_last) {
   if (need_lattice(h)) return 1;
   for (int q = 0; q < nsteps; q++)
-    if (do_step(h, write_macro_last && q == nsteps - 1)) return 1;
+    if (do_step(h, q == nsteps - 1 ? write_macro_last : 0)) return 1;
   return sync(h);
 }
 
@@ -722,6 +722,23 @@ extern "C" int lbm_download_macro(lbm_handle *h, int subs, double *rho, double *
     memset(dst[c], 0, sizeof(double) * h->N);
     if (c > D) continue;
     for (int x = 0; x < nf; x++) dst[c][h->node_host[x]] = buf[(size_t)c * nf + x];
+  }
+  return 0;
+}
+
+extern "C" int lbm_upload_macro(lbm_handle *h, int subs, const double *rho, const double *ux, const double *uy,
+                                const double *uz) {
+  if (need_lattice(h)) return 1;
+  if (subs < 0 || subs >= h->p.nsubs) return lbm_fail("bad subs %d", subs);
+  HIP_OK(hipStreamSynchronize(h->stream));
+  const int nf = h->nf, D = h->D;
+  double *mv = h->mv + (long)subs * (1 + D) * nf;
+  const double *src[4] = {rho, ux, uy, uz};
+  std::vector<double> buf((size_t)nf);
+  for (int c = 0; c <= D; c++) {
+    if (!src[c]) continue;
+    for (int x = 0; x < nf; x++) buf[x] = src[c][h->node_host[x]];
+    HIP_OK(hipMemcpy(mv + (long)c * nf, buf.data(), (size_t)nf * sizeof(double), hipMemcpyHostToDevice));
   }
   return 0;
 }

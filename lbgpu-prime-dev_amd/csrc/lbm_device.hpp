@@ -239,13 +239,20 @@ __device__ __forceinline__ void collide(const StepArgs<ST, CT> &A, int idx, CT (
     rho *= A.sink_fac;
   }
   if (A.write_mv) {
+    // write_mv 2 is the PEST observation switch: flow u only, solute c only
+    // (src/k_stream_collide_stream.c:305-334)
     double *mv = A.mv + (long)SUBS * (1 + D) * A.nf;
-    if (SUBS == 0)
-      mv[idx] = (double)(rho + A.rhoA);
-    else
-      mv[idx] = (double)rho;
+    const bool part = A.write_mv == 2;
+    if (!part || SUBS == 1) {
+      if (SUBS == 0)
+        mv[idx] = (double)(rho + A.rhoA);
+      else
+        mv[idx] = (double)rho;
+    }
+    if (!part || SUBS == 0) {
 #pragma unroll
-    for (int d = 0; d < D; d++) mv[(long)(d + 1) * A.nf + idx] = (double)u[d];
+      for (int d = 0; d < D; d++) mv[(long)(d + 1) * A.nf + idx] = (double)u[d];
+    }
   }
   CT usq = (CT)0;
   if (SUBS == 0) {

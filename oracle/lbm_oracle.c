@@ -299,7 +299,14 @@ static void update_node(orc *o, int odd, int t, int write_mv, int i, int j, int 
     } else {
       rho *= (1.0 - p->sink * p->tau[1]);
     }
-    if (write_mv) {
+    if (write_mv == 2) {
+      /* pest_output_flag_c off a frame end: flow u only, solute c only
+       * (src/k_stream_collide_stream.c:305-334, src/k_collide.c:313-340) */
+      if (s == 0)
+        for (a = 0; a < D; a++) set_mv(o, s, n, a + 1, u[a]);
+      else
+        set_mv(o, s, n, 0, rho);
+    } else if (write_mv) {
       set_mv(o, s, n, 0, s == 0 ? rho + p->rho_A[s] : rho);
       for (a = 0; a < D; a++) set_mv(o, s, n, a + 1, u[a]);
     }

@@ -87,7 +87,9 @@ void orc_set_time(orc *o, int t);
 int orc_get_time(const orc *o);
 
 /* One LBM step: time <- time+1; odd times run stream-collide-stream, even
- * times collide.  write_mv stores the pre-collision moments. */
+ * times collide.  write_mv 1 stores the pre-collision moments (frame
+ * end); 2 stores only the flow velocity and the solute concentration (the
+ * PEST observation switch). */
 void orc_step(orc *o, int write_mv);
 void orc_run(orc *o, int nsteps);
 

@@ -208,11 +208,11 @@ class Oracle:
         lib().orc_set_time(self.h, t)
 
     def step(self, write_mv=False):
-        lib().orc_step(self.h, int(bool(write_mv)))
+        lib().orc_step(self.h, int(write_mv))
 
     def run(self, n, write_last=True):
         for q in range(n):
-            self.step(write_last and q == n - 1)
+            self.step(write_last if q == n - 1 else 0)
 
     def mv(self, subs=0):
         rho, ux, uy, uz = (np.empty(self.N) for _ in range(4))

@@ -155,6 +155,45 @@ def test_3d_zface_pressure_solute_flux(halo):
     assert relnorm(sg, so) <= 1e-12
 
 
+@pytest.mark.parametrize("dims", [2, 3])
+def test_pest_partial_macro_write_and_upload(dims):
+    """LBM_MACRO_PEST writes the flow velocity and the solute concentration
+    only; the rest of the buffer keeps what lbm_upload_macro put there"""
+    say(f"case PEST macro write dims={dims}")
+    rng = np.random.default_rng(31 + dims)
+    shape = (11, 13) if dims == 2 else (6, 7, 9)
+    mask = (rng.random(shape) < 0.25).astype(np.uint8)
+    kw = dict(dims=dims, ni=shape[-1], nj=shape[-2], nk=shape[0] if dims == 3 else 1, nsubs=2,
+              tau=(0.8, 0.7), gaccel=(1e-4, -2e-4, 3e-4, 0, 0, 0), sink=1e-3)
+    o, g = build_pair(mask, **kw)
+    f, gg = random_state(rng, 9 if dims == 2 else 19, mask.size, 2)
+    load_both(o, g, f, gg)
+    o.run(3)
+    g.run(3)
+    compare(o, g, mask, 2, "before PEST steps")
+    fl = mask.reshape(-1) != 1
+    N = mask.size
+    sentinel = [rng.uniform(-1, 1, N) for _ in range(8)]
+    g.set_mv(0, *sentinel[0:4])
+    g.set_mv(1, *sentinel[4:8])
+    back = g.mv(0) + g.mv(1)
+    for c in range(1 + dims):
+        assert np.array_equal(back[c][fl], sentinel[c][fl])
+        assert np.array_equal(back[4 + c][fl], sentinel[4 + c][fl])
+        assert not back[c][~fl].any()
+    for t in range(2):  # one odd and one even half-step
+        o.step(2)
+        g.step(2)
+        mo0, mo1, mg0, mg1 = o.mv(0), o.mv(1), g.mv(0), g.mv(1)
+        assert np.array_equal(mg0[0][fl], sentinel[0][fl])
+        for c in range(1, 1 + dims):
+            assert np.array_equal(mg0[c][fl], mo0[c][fl])
+            assert np.array_equal(mg1[c][fl], sentinel[4 + c][fl])
+        assert np.array_equal(mg1[0][fl], mo1[0][fl])
+    for s in range(2):
+        assert np.array_equal(g.download_f(s)[:, fl], o.get_f(s)[:, fl])
+
+
 class DeviceBuffer:
     """device memory from the HIP runtime the library already loaded"""
 

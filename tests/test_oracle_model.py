@@ -93,3 +93,28 @@ def test_openmp_equals_serial_bitwise():
     pyoracle.set_threads(0)
     for a, b in zip(res[0], res[1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("dims,shape", [(2, (7, 9)), (3, (5, 6, 7))])
+def test_pest_write_selects_components(dims, shape):
+    """write_mv 2 (the PEST observation switch) stores the flow velocity and
+    the solute concentration of a full frame write, keeps the flow density
+    and the solute velocity from before, and steps the populations exactly
+    as a step without writes"""
+    mask, f, g = random_case(dims, shape, 2, seed=50 + dims)
+    a = make(dims, shape, 2, (0.8, 0.7), (1e-4, 2e-4, -1e-4), mask, f, g)
+    b = make(dims, shape, 2, (0.8, 0.7), (1e-4, 2e-4, -1e-4), mask, f, g)
+    a.run(3)
+    b.run(3)
+    before = (b.mv(0), b.mv(1))
+    for _ in range(2):
+        a.step(1)
+        b.step(2)
+        fa, fb = (a.mv(0), a.mv(1)), (b.mv(0), b.mv(1))
+        assert np.array_equal(fb[0][0], before[0][0])
+        assert np.array_equal(fb[1][0], fa[1][0])
+        for c in range(1, 1 + dims):
+            assert np.array_equal(fb[0][c], fa[0][c])
+            assert np.array_equal(fb[1][c], before[1][c])
+    for s in range(2):
+        assert np.array_equal(a.get_f(s), b.get_f(s))
