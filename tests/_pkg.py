@@ -18,3 +18,10 @@ def load():
     sys.modules[name] = mod
     spec.loader.exec_module(mod)
     return mod
+
+
+def load_submodule(name):
+    """a module of the package (e.g. "deck")"""
+    import importlib
+    load()
+    return importlib.import_module("lbgpu_prime_dev_amd." + name)

@@ -14,18 +14,24 @@ ROOT = _pkg.ROOT
 
 
 def header_symbols():
-    src = open(os.path.join(ROOT, "include", "lbm_mi355x.h")).read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(lbm_[a-z0-9_]+)\s*\(", src)))
+    syms = set()
+    inc = os.path.join(ROOT, "include")
+    for name in sorted(os.listdir(inc)):
+        if name.endswith(".h"):
+            src = open(os.path.join(inc, name)).read()
+            src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+            syms |= set(re.findall(r"\b(lbm_[a-z0-9_]+)\s*\(", src))
+    return sorted(syms)
 
 
 def test_library_exports_header_symbols():
     L = _pkg.load()
     lib = L.lib()
     syms = header_symbols()
-    assert len(syms) >= 25
+    assert len(syms) >= 40
     for s in syms:
         assert hasattr(lib, s), s
+    assert "lbm_io_read_deck" in syms and "lbm_deck_main" in syms
 
 
 def test_discs2d_properties_and_hash():
