@@ -154,6 +154,23 @@ int lbm_io_write_dat(const char *path, const double *a, const uint8_t *mask, lon
 int lbm_io_write_raw8(const char *path, const double *ux, const double *uy, const double *uz,
                       const uint8_t *mask, long N, const double a_max[3], const double a_min[3]);
 
+/* read_PEST_in_files (src/latman.c:889-1038): observation times from
+ * t_path (integers, counted as the pressure series are), and the x and y
+ * lattice coordinates of each from x_path and y_path.  Times that decrease
+ * are an error ("concentration data not in time order" there).  Points
+ * outside 0..LX-1 x 0..LY-1 are dropped; order[] keeps each kept point's
+ * position in the files.  With t == NULL only *count (kept points) is
+ * returned; otherwise at most cap points are stored. */
+int lbm_io_read_observations(const char *t_path, const char *x_path, const char *y_path, int LX, int LY, int cap,
+                             int *t, int *x, int *y, int *order, int *count);
+
+/* write_PEST_out_file (src/latman.c:1196-1255) of process 0: under
+ * out_dir, conc_data_proc0000.dat and vel_data_proc0000.dat ("%20.17f" per
+ * point: the solute concentration and the flow speed |(u_x, u_y)| at the
+ * observation) and conc_order_proc0000.dat ("%d": the point's order). */
+int lbm_io_write_observations(const char *out_dir, int count, const double *conc, const double *vel,
+                              const int *order);
+
 /* main() of the reference (src/lbgpu_prime.cu:92-915) on this library:
  * argv as for the lbgpu_prime executable ([name] [--dims 2|3] [--subs 1|2]
  * [--storage f64|f32|f16] [--compute f64|f32] [--in DIR] [--out DIR]).
@@ -161,8 +178,11 @@ int lbm_io_write_raw8(const char *path, const double *ux, const double *uy, cons
  * solids (<in>/LXxLY.bmp in 2-D, <in>/LXxLYxLZ.raw in 3-D) and the pressure
  * series, starts from init_problem's rest state (initial_condition 0 or 2),
  * writes frame 0, steps NumFrames * FrameRate times and writes each frame
- * (frames.dat, .dat, 3-D .raw, rho/u .txt) under <out>.  Returns 0, or 1
- * after printing the reason (process_exit(1) there). */
+ * (frames.dat, .dat, 3-D .raw, rho/u .txt) under <out>.  With
+ * <in>/timestep_file.in present, the observation points of
+ * <in>/{timestep,x_coord,y_coord}_file.in are sampled on their steps (the
+ * PEST switch) and written at the end (lbm_io_write_observations).
+ * Returns 0, or 1 after printing the reason (process_exit(1) there). */
 int lbm_deck_main(int argc, char **argv);
 
 /* sizeof(lbm_deck), for bindings that mirror the struct */

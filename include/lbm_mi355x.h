@@ -150,6 +150,12 @@ int lbm_download_macro(lbm_handle *h, int subs, double *rho, double *ux, double 
 int lbm_upload_macro(lbm_handle *h, int subs, const double *rho, const double *ux, const double *uy,
                      const double *uz);
 
+/* mv at single nodes (the per-node transfers of write_PEST_out_data,
+ * src/latman.c:1130-1170): for each dense node index in nodes[npts], out
+ * receives rho (c for subs 1) and the D velocity components, 1 + D values
+ * per node; solid nodes give 0. */
+int lbm_sample_macro(lbm_handle *h, int subs, int npts, const long *nodes, double *out);
+
 /* FP64 reductions on the GPU over the last macroscopic write */
 int lbm_reduce(lbm_handle *h, int kind, int subs, double *out);
 /* solute moments of plane flux_plane on the last step with a macroscopic

@@ -88,6 +88,7 @@ def lib():
             "lbm_halo_periodic": [vp, ip],
             "lbm_download_macro": [vp, ip, dp, dp, dp, dp],
             "lbm_upload_macro": [vp, ip, dp, dp, dp, dp],
+            "lbm_sample_macro": [vp, ip, ip, ctypes.POINTER(ctypes.c_long), dp],
             "lbm_reduce": [vp, ip, ip, dp],
             "lbm_last_flux": [vp, dp, dp],
             "lbm_info": [vp, lp, lp, lp],
@@ -279,6 +280,14 @@ class Lattice:
         rho, ux, uy, uz = (np.empty(self.N) for _ in range(4))
         _check(lib().lbm_download_macro(self.h, subs, _dp(rho), _dp(ux), _dp(uy), _dp(uz)))
         return rho, ux, uy, uz
+
+    def sample_mv(self, subs, nodes):
+        """lbm_sample_macro: [len(nodes), 1 + dims] values of mv at dense nodes"""
+        nodes = np.ascontiguousarray(nodes, dtype=np.int64).reshape(-1)
+        out = np.empty((nodes.size, 1 + self.dims))
+        _check(lib().lbm_sample_macro(self.h, subs, nodes.size,
+                                      nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), _dp(out)))
+        return out
 
     def set_mv(self, subs, rho=None, ux=None, uy=None, uz=None):
         """lbm_upload_macro; None leaves a component as it is"""

@@ -726,6 +726,27 @@ extern "C" int lbm_download_macro(lbm_handle *h, int subs, double *rho, double *
   return 0;
 }
 
+extern "C" int lbm_sample_macro(lbm_handle *h, int subs, int npts, const long *nodes, double *out) {
+  if (need_lattice(h)) return 1;
+  if (subs < 0 || subs >= h->p.nsubs) return lbm_fail("bad subs %d", subs);
+  if (npts < 0 || (npts && (!nodes || !out))) return lbm_fail("bad sample arguments");
+  HIP_OK(hipStreamSynchronize(h->stream));
+  const int nf = h->nf, D = h->D;
+  const double *mv = h->mv + (long)subs * (1 + D) * nf;
+  for (int q = 0; q < npts; q++) {
+    if (nodes[q] < 0 || nodes[q] >= h->N) return lbm_fail("sample node %ld outside the lattice", nodes[q]);
+    const int x = h->fid[nodes[q]];
+    for (int c = 0; c <= D; c++) {
+      double *dst = out + (long)q * (1 + D) + c;
+      if (x < 0)
+        *dst = 0.0;
+      else
+        HIP_OK(hipMemcpy(dst, mv + (long)c * nf + x, sizeof(double), hipMemcpyDeviceToHost));
+    }
+  }
+  return 0;
+}
+
 extern "C" int lbm_upload_macro(lbm_handle *h, int subs, const double *rho, const double *ux, const double *uy,
                                 const double *uz) {
   if (need_lattice(h)) return 1;

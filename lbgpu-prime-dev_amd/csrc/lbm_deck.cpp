@@ -273,6 +273,35 @@ int deck_main(int argc, char **argv) {
                            R.dims == 3 ? R.mv[s].uz.data() : nullptr),
           "mv H2D");
 
+  // PEST observation points (read_PEST_in_files, src/latman.c:889-1038),
+  // used when the deck has a timestep file
+  const std::string obs_t = R.in_dir + "/timestep_file.in";
+  FILE *probe_obs = fopen(obs_t.c_str(), "r");
+  const bool pest = probe_obs != nullptr;
+  if (probe_obs) fclose(probe_obs);
+  std::vector<int> ot, ox, oy, oorder;
+  std::vector<double> oconc, ovel;
+  if (pest) {
+    const std::string xs = R.in_dir + "/x_coord_file.in", ys = R.in_dir + "/y_coord_file.in";
+    int cnt = 0;
+    check(lbm_io_read_observations(obs_t.c_str(), xs.c_str(), ys.c_str(), d.LX, d.LY, 0, nullptr, nullptr,
+                                   nullptr, nullptr, &cnt),
+          "read_PEST_in_files");
+    ot.resize(cnt);
+    ox.resize(cnt);
+    oy.resize(cnt);
+    oorder.resize(cnt);
+    check(lbm_io_read_observations(obs_t.c_str(), xs.c_str(), ys.c_str(), d.LX, d.LY, cnt, ot.data(), ox.data(),
+                                   oy.data(), oorder.data(), &cnt),
+          "read_PEST_in_files");
+    oconc.assign(cnt, 0.0);
+    ovel.assign(cnt, 0.0);
+    printf("Number of PEST points = %d\n", cnt);
+  }
+  size_t opos = 0;
+  const int D = R.dims;
+  std::vector<double> sample(1 + D);
+
   // time loop (src/lbgpu_prime.cu:369-915): the odd and even half-steps,
   // frame output after the half-step that ends a frame, stop at the final
   // frame
@@ -281,9 +310,30 @@ int deck_main(int argc, char **argv) {
   int frame = 0, steps = 0;
   for (int time = 1; time <= d.NumTimeSteps; time++) {
     const int end_of_frame = d.FrameRate > 0 && time % d.FrameRate == 0;
-    check(lbm_step(h, end_of_frame ? LBM_MACRO_FRAME : LBM_MACRO_NONE), time % 2 ? "k_stream_collide_stream"
-                                                                                 : "k_collide");
+    const bool observe = pest && opos < ot.size() && ot[opos] == time;
+    const int macro = end_of_frame ? LBM_MACRO_FRAME : observe ? LBM_MACRO_PEST : LBM_MACRO_NONE;
+    check(lbm_step(h, macro), time % 2 ? "k_stream_collide_stream" : "k_collide");
     steps++;
+    // write_PEST_out_data (src/latman.c:1072-1190): the solute
+    // concentration and flow speed at each point observed at this time
+    while (pest && opos < ot.size() && ot[opos] == time) {
+      const long n = (long)oy[opos] * d.LX + ox[opos];
+      double c = 0.0, ux = 0.0, uy = 0.0;
+      if (R.mask[n] == 1) {
+        if (R.nsubs == 2) c = R.init_mv[1].rho[n];
+      } else {
+        if (R.nsubs == 2) {
+          check(lbm_sample_macro(h, 1, 1, &n, sample.data()), "PEST transfer");
+          c = sample[0];
+        }
+        check(lbm_sample_macro(h, 0, 1, &n, sample.data()), "PEST transfer");
+        ux = sample[1];
+        uy = sample[2];
+      }
+      oconc[opos] = c;
+      ovel[opos] = sqrt(ux * ux + uy * uy);
+      opos++;
+    }
     if (!end_of_frame) continue;
     auto a = std::chrono::steady_clock::now();
     ++frame;
@@ -311,6 +361,9 @@ int deck_main(int argc, char **argv) {
     }
   }
   const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (pest)
+    check(lbm_io_write_observations(R.out_dir.c_str(), (int)ot.size(), oconc.data(), ovel.data(), oorder.data()),
+          "write_PEST_out_file");
   const double comp = total - io_s;
   printf("lbgpu_prime: %d steps of %ld nodes in %.3f s (%.3f s stepping, %.3f s file I/O): %.1f MLUPS\n", steps,
          R.N, total, comp, io_s, comp > 0 ? (double)R.N * steps / comp / 1e6 : 0.0);

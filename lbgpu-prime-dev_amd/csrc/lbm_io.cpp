@@ -147,6 +147,8 @@ inline uint8_t byte_of(double x) {
   return (uint8_t)(i & 0xff);
 }
 
+int count_items(FILE *in, const char *fmt, const char *path, int *n);
+
 }  // namespace
 
 extern "C" int lbm_io_read_deck(const char *path, int dims, int nsubs, lbm_deck *d) {
@@ -244,12 +246,7 @@ extern "C" int lbm_io_read_series(const char *path, double *values, int cap, int
   // count as src/latman.c:552-558 does: one scan ahead, count while not at
   // end of file
   int n = 0;
-  double v;
-  (void)fscanf(in.f, "%lf", &v);
-  while (!feof(in.f)) {
-    n++;
-    if (fscanf(in.f, "%lf", &v) != 1 && !feof(in.f)) return lbm_fail("%s: not a number after %d values", path, n);
-  }
+  if (count_items(in.f, "%lf", path, &n)) return 1;
   *count = n;
   if (!values) return 0;
   rewind(in.f);
@@ -469,5 +466,83 @@ extern "C" int lbm_io_write_raw8(const char *path, const double *ux, const doubl
   File o(fopen(path, "wb"));
   if (!o.f) return lbm_fail("Can't creat file \"%s\"", path);
   if (fwrite(buf.data(), 1, (size_t)N, o.f) != (size_t)N) return lbm_fail("%s: short write", path);
+  return 0;
+}
+
+namespace {
+
+// the feof counting idiom of the reference's list readers: one scan ahead,
+// count while not at the end of the file
+int count_items(FILE *in, const char *fmt, const char *path, int *n) {
+  union {
+    int i;
+    double d;
+  } v;
+  int c = 0;
+  (void)fscanf(in, fmt, &v);
+  while (!feof(in)) {
+    c++;
+    if (fscanf(in, fmt, &v) != 1 && !feof(in)) return lbm_fail("%s: unreadable entry after %d values", path, c);
+  }
+  *n = c;
+  return 0;
+}
+
+int read_ints(const char *path, int n, std::vector<int> &out) {
+  File in(fopen(path, "r"));
+  if (!in.f) return lbm_fail("can't load \"%s\"", path);
+  out.resize(n);
+  for (int q = 0; q < n; q++)
+    if (fscanf(in.f, "%d", &out[q]) != 1) return lbm_fail("%s: %d values expected, %d read", path, n, q);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int lbm_io_read_observations(const char *t_path, const char *x_path, const char *y_path, int LX, int LY,
+                                        int cap, int *t, int *x, int *y, int *order, int *count) {
+  if (!t_path || !x_path || !y_path || !count) return lbm_fail("null argument");
+  int n = 0;
+  {
+    File in(fopen(t_path, "r"));
+    if (!in.f) return lbm_fail("can't load \"%s\"", t_path);
+    if (count_items(in.f, "%d", t_path, &n)) return 1;
+  }
+  std::vector<int> tt, xx, yy;
+  if (read_ints(t_path, n, tt)) return 1;
+  for (int q = 1; q < n; q++)
+    if (tt[q] < tt[q - 1]) return lbm_fail("%s: observation times not in time order (entry %d)", t_path, q);
+  if (read_ints(x_path, n, xx) || read_ints(y_path, n, yy)) return 1;
+  int kept = 0;
+  for (int q = 0; q < n; q++) {
+    if (xx[q] < 0 || xx[q] > LX - 1 || yy[q] < 0 || yy[q] > LY - 1) continue;
+    if (t && kept < cap) {
+      t[kept] = tt[q];
+      if (x) x[kept] = xx[q];
+      if (y) y[kept] = yy[q];
+      if (order) order[kept] = q;
+    }
+    kept++;
+  }
+  *count = kept;
+  return 0;
+}
+
+extern "C" int lbm_io_write_observations(const char *out_dir, int count, const double *conc, const double *vel,
+                                         const int *order) {
+  if (!out_dir || count < 0 || (count && (!conc || !vel || !order))) return lbm_fail("bad argument");
+  const char *names[3] = {"conc_data", "vel_data", "conc_order"};
+  for (int f = 0; f < 3; f++) {
+    char path[1024];
+    int len = snprintf(path, sizeof path, "%s/%s_proc%04d.dat", out_dir, names[f], 0);
+    if (len < 0 || (size_t)len >= sizeof path) return lbm_fail("file name too long");
+    File o(fopen(path, "w+"));
+    if (!o.f) return lbm_fail("can't write \"%s\"", path);
+    for (int q = 0; q < count; q++) {
+      if (f == 0) fprintf(o.f, "%20.17f\n", conc[q]);
+      if (f == 1) fprintf(o.f, "%20.17f\n", vel[q]);
+      if (f == 2) fprintf(o.f, "%d\n", order[q]);
+    }
+  }
   return 0;
 }

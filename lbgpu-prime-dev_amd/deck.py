@@ -77,6 +77,10 @@ def lib():
             "lbm_io_write_dat": [cp, dp, u8p, ctypes.c_long],
             "lbm_io_write_raw8": [cp, dp, dp, dp, u8p, ctypes.c_long, dp, dp],
             "lbm_deck_main": [ip, ctypes.POINTER(ctypes.c_char_p)],
+            "lbm_io_read_observations": [cp, cp, cp, ip, ip, ip, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+            "lbm_io_write_observations": [cp, ip, dp, dp, ctypes.POINTER(ctypes.c_int)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -179,6 +183,27 @@ def write_raw8(path, ux, mask, a_max, a_min, uy=None, uz=None):
     m = np.ascontiguousarray(mask, dtype=np.uint8).reshape(-1)
     hi, lo = _f64(np.resize(np.asarray(a_max, float), 3)), _f64(np.resize(np.asarray(a_min, float), 3))
     _check(lib().lbm_io_write_raw8(_b(path), _dp(ux), _dp(uy), _dp(uz), _u8(m), m.size, _dp(hi), _dp(lo)))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def read_observations(t_path, x_path, y_path, LX, LY):
+    """(times, x, y, order) of the kept observation points"""
+    n = ctypes.c_int()
+    _check(lib().lbm_io_read_observations(_b(t_path), _b(x_path), _b(y_path), LX, LY, 0, None, None, None,
+                                          None, ctypes.byref(n)))
+    arrs = [np.zeros(max(n.value, 1), np.int32) for _ in range(4)]
+    _check(lib().lbm_io_read_observations(_b(t_path), _b(x_path), _b(y_path), LX, LY, n.value,
+                                          *(_ip(a) for a in arrs), ctypes.byref(n)))
+    return tuple(a[:n.value] for a in arrs)
+
+
+def write_observations(out_dir, conc, vel, order):
+    conc, vel = _f64(conc), _f64(vel)
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    _check(lib().lbm_io_write_observations(_b(out_dir), order.size, _dp(conc), _dp(vel), _ip(order)))
 
 
 def deck_main(*args):

@@ -320,3 +320,32 @@ def test_deck_main_argument_errors(io, tmp_path):
     assert io.deck_main("--dims", "4") == 1
     assert io.deck_main("--in", str(tmp_path / "missing")) == 1
     assert io.deck_main("--help") == 0
+
+
+def test_observations_read_filter_and_order(io, tmp_path):
+    t, x, y = tmp_path / "t.in", tmp_path / "x.in", tmp_path / "y.in"
+    t.write_text("5\n10\n10\n20\n31\n")
+    x.write_text("1 100 3 4 9\n")
+    y.write_text("2 2 5 -1 7\n")
+    ot, ox, oy, order = io.read_observations(t, x, y, 10, 8)
+    # index 1 (x = 100) and index 3 (y = -1) lie outside the 10 x 8 lattice
+    assert list(ot) == [5, 10, 31] and list(ox) == [1, 3, 9] and list(oy) == [2, 5, 7]
+    assert list(order) == [0, 2, 4]
+    t.write_text("5 10 20")  # no newline after the last time: it is not counted
+    ot, ox, oy, order = io.read_observations(t, x, y, 10, 8)
+    assert list(ot) == [5] and list(order) == [0]  # the second point is outside
+    t.write_text("5\n10\n7\n")
+    with pytest.raises(Exception, match="time order"):
+        io.read_observations(t, x, y, 10, 8)
+
+
+def test_observations_written(io, tmp_path):
+    conc = np.array([0.25, 1.0 / 3.0, 0.0])
+    vel = np.array([1e-5, 2.5e-3, 0.0])
+    io.write_observations(tmp_path, conc, vel, [0, 2, 4])
+    c = (tmp_path / "conc_data_proc0000.dat").read_text().split("\n")
+    v = (tmp_path / "vel_data_proc0000.dat").read_text().split("\n")
+    o = (tmp_path / "conc_order_proc0000.dat").read_text().split("\n")
+    assert c[:3] == [" 0.25000000000000000", " 0.33333333333333331", " 0.00000000000000000"]
+    assert [float(q) for q in v[:3]] == [1e-5, 2.5e-3, 0.0] and len(v[0]) == 20
+    assert o == ["0", "2", "4", ""]

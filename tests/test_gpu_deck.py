@@ -78,6 +78,13 @@ def test_deck_run_matches_oracle(tmp_path, nsubs):
              "bc_ramp_start": 2, "bc_ramp_stop": 9, "sigma_t_on": 0, "sigma_t_off": -1,
              "x1": 10, "x2": 20, "y1": 8, "y2": 20, "rho_B": "1.01", "rho_sigma": "1.0", "sink": "0.001"}
     write_deck(str(ind / "params.in"), edits)
+    # PEST observation points: odd, even and frame-end times, two points at
+    # one time, one solid point and one outside the lattice (dropped)
+    sy, sx = np.argwhere(mask == 1)[0]
+    obs = [(3, 5, 6), (4, 30, 20), (4, 12, 14), (7, 40, 3), (9, int(sx), int(sy)), (11, 60, 4), (16, 15, 10)]
+    (ind / "timestep_file.in").write_text("".join(f"{t}\n" for t, _, _ in obs))
+    (ind / "x_coord_file.in").write_text(" ".join(str(x) for _, x, _ in obs) + "\n")
+    (ind / "y_coord_file.in").write_text(" ".join(str(y) for _, _, y in obs) + "\n")
     out = tmp_path / "out"
     out.mkdir()
     rc = io.deck_main("--dims", "2", "--subs", str(nsubs), "--in", str(ind), "--out", str(out))
@@ -109,9 +116,25 @@ def test_deck_run_matches_oracle(tmp_path, nsubs):
         z = np.zeros(LX * LY)
         expected_rows[s].append(io.frame_stats(rho0[s], z, z, z, mask))
         assert np.max(np.abs(read_txt(name("rho", 0, s, ".txt"), LX, LY) - rho0[s])) <= 1e-15
+    obs_times = {t for t, _, _ in obs}
+    conc, vel, order = [], [], []
+    time = 0
     for frame in range(1, NF + 1):
-        for t in range(FR):
-            o.step(t == FR - 1)
+        for _ in range(FR):
+            time += 1
+            o.step(1 if time % FR == 0 else (2 if time in obs_times else 0))
+            for q, (t, x, y) in enumerate(obs):
+                if t != time or x >= LX:
+                    continue
+                n = y * LX + x
+                if solid[n]:
+                    c, ux, uy = rho0[1][n] if nsubs == 2 else 0.0, 0.0, 0.0
+                else:
+                    c = o.mv(1)[0][n] if nsubs == 2 else 0.0
+                    ux, uy = o.mv(0)[1][n], o.mv(0)[2][n]
+                conc.append(c)
+                vel.append(np.sqrt(ux * ux + uy * uy))
+                order.append(q)
         for s in range(nsubs):
             mo = o.mv(s)
             fields = {"rho": mo[0], "u_x": mo[1], "u_y": mo[2]}
@@ -134,6 +157,12 @@ def test_deck_run_matches_oracle(tmp_path, nsubs):
             want = [r[0], 0, 0, 0, 0] + list(mn) + list(mx) + list(av)
             assert np.allclose(r[1:14], want[1:14], rtol=0, atol=6e-8)
             assert np.allclose(r[17:20], [st[1], st[0], st[2]], rtol=0, atol=6e-8)
+    got_c = [float(v) for v in (out / "conc_data_proc0000.dat").read_text().split()]
+    got_v = [float(v) for v in (out / "vel_data_proc0000.dat").read_text().split()]
+    got_o = [int(v) for v in (out / "conc_order_proc0000.dat").read_text().split()]
+    assert got_o == order == [0, 1, 2, 3, 4, 6]
+    assert np.allclose(got_c, conc, rtol=0, atol=1e-17) and np.allclose(got_v, vel, rtol=0, atol=1e-17)
+    assert max(vel) > 0 and (nsubs == 1 or max(conc) > 0)
     if nsubs == 2:
         # the solute left the cube and reached the fluid around it
         assert o.mv(1)[0][fluid & ~cube.reshape(-1)].max() > 0

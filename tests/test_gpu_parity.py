@@ -192,6 +192,14 @@ def test_pest_partial_macro_write_and_upload(dims):
         assert np.array_equal(mg1[0][fl], mo1[0][fl])
     for s in range(2):
         assert np.array_equal(g.download_f(s)[:, fl], o.get_f(s)[:, fl])
+    # single-node transfers (lbm_sample_macro) equal the full download;
+    # solid nodes give 0
+    nodes = np.array([0, N - 1, int(np.argmax(~fl)), int(np.argmax(fl)), N // 2])
+    for s in range(2):
+        full = g.mv(s)
+        got = g.sample_mv(s, nodes)
+        for c in range(1 + dims):
+            assert np.array_equal(got[:, c], full[c][nodes])
 
 
 class DeviceBuffer:
