@@ -79,21 +79,44 @@ def adjoint_timing(L, n=128, steps=2000, window=200, every=100):
     lat = L.Lattice(mask=mask, dims=3, ni=n, nj=n, nk=n, nsubs=2, tau=(0.8, 0.75),
                     gaccel=(0.0, 0.0, 1e-6, 0, 0, 0), zface_pressure=1, rho_bot=1.001, rho_top=1.0,
                     zface_solute=1, c_bot_map=cmap, flux_plane=n - 2)
-    lat.init_rest()
-    lat.run(2, write_last=False)
-    lat.init_rest()
-    lat.time = 0
+    def start():
+        lat.init_rest()
+        lat.time = 0
+
+    # warm-up pair: allocates the checkpoints and the adjoint workspace,
+    # which the library keeps for the next run
+    start()
+    lat.objective(steps, window, every)
+    lat.adjoint()
+    # a plain forward run of the same length, for reference
+    start()
+    t0 = time.perf_counter()
+    lat.run(steps, write_last=True)
+    t_plain = time.perf_counter() - t0
+    # the timed pair
+    start()
     t0 = time.perf_counter()
     J = lat.objective(steps, window, every)
     t_fwd = time.perf_counter() - t0
     t0 = time.perf_counter()
     grad, gmap = lat.adjoint()
     t_adj = time.perf_counter() - t0
+    # where the adjoint sweep spends its time (events on every launch)
+    lat.profile(True)
+    lat.profile_read()
+    lat.adjoint()
+    prof = lat.profile_read()
+    lat.profile(False)
     lat.close()
-    log(f"[adjoint] J {J:.6e} forward {t_fwd:.3f}s adjoint {t_adj:.3f}s grad {grad}")
+    replay_ms = prof["odd"][0] + prof["even"][0]
+    adj_ms = prof["adjoint"][0]
+    log(f"[adjoint] J {J:.6e} forward {t_fwd:.3f}s (plain run {t_plain:.3f}s) adjoint {t_adj:.3f}s "
+        f"(replay {replay_ms:.1f} ms, adjoint kernels {adj_ms:.1f} ms, bc {prof['bc'][0]:.1f} ms) grad {grad}")
     return {"lattice": [n, n, n], "porosity": round(phi, 5), "steps": steps, "window": window,
             "checkpoint_every": every, "J": J, "dJ_dgz": grad[2], "dJ_dtau": grad[3], "dJ_dtau_s": grad[4],
-            "dJ_dcmap_sum": float(gmap.sum()), "forward_s": round(t_fwd, 4), "adjoint_s": round(t_adj, 4),
+            "dJ_dcmap_sum": float(gmap.sum()), "forward_s": round(t_fwd, 4), "plain_forward_s": round(t_plain, 4),
+            "adjoint_s": round(t_adj, 4), "adjoint_replay_ms": round(replay_ms, 2),
+            "adjoint_kernels_ms": round(adj_ms, 2),
             "adjoint_over_forward": round(t_adj / t_fwd, 3),
             "gradient_over_forward": round((t_fwd + t_adj) / t_fwd, 3)}
 

@@ -168,9 +168,12 @@ int lbm_info(const lbm_handle *h, long *n_nodes, long *n_fluid, long *row_stride
 
 /* Kernel timing with HIP events on the library's stream.  While enabled
  * every launch is bracketed by events; lbm_profile_read returns the summed
- * milliseconds and launch counts of the kernel classes
+ * milliseconds and launch counts of the LBM_PROFILE_CLASSES kernel classes
  * [0] odd half-step, [1] even half-step, [2] boundary conditions,
- * [3] halo pack/unpack, [4] plane flux, and resets them. */
+ * [3] halo pack/unpack, [4] plane flux, [5] adjoint step (with its
+ * boundary transposes; the adjoint's forward replay counts as [0]/[1]),
+ * and resets them.  ms and count hold LBM_PROFILE_CLASSES entries. */
+#define LBM_PROFILE_CLASSES 6
 int lbm_profile_enable(lbm_handle *h, int on);
 int lbm_profile_read(lbm_handle *h, double *ms, int *count);
 
@@ -185,7 +188,11 @@ int lbm_objective_run(lbm_handle *h, int nsteps, int window, int every, double *
 /* Gradient of the last objective by the discrete adjoint, replaying each
  * segment between saved states out of place and sweeping it backwards:
  * grad[0..4] = dJ/d(g_x, g_y, g_z, tau, tau_s) of the flow's body force
- * and relaxation times; grad_cmap (optional, [LY][LX]) = dJ/dc_bot_map. */
+ * and relaxation times; grad_cmap (optional, [LY][LX]) = dJ/dc_bot_map.
+ * Device memory: nsteps/every saved states (objective run) and every + 2
+ * states (adjoint run), each the size of the population array; both stay
+ * allocated for the next run and are released by lbm_set_solids and
+ * lbm_destroy. */
 int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap);
 
 /* Device-to-device copy of nbytes (reps times) on a stream of its own:

@@ -22,7 +22,7 @@ STORE_F64, STORE_F32, STORE_F16 = 0, 1, 2
 COMPUTE_F64, COMPUTE_F32 = 0, 1
 REDUCE_REF, REDUCE_SUMS = 0, 1
 MACRO_NONE, MACRO_FRAME, MACRO_PEST = 0, 1, 2
-PROFILE_CLASSES = ("odd", "even", "bc", "halo", "flux")
+PROFILE_CLASSES = ("odd", "even", "bc", "halo", "flux", "adjoint")
 
 
 class LbmError(RuntimeError):
@@ -330,8 +330,8 @@ class Lattice:
         _check(lib().lbm_profile_enable(self.h, int(bool(on))))
 
     def profile_read(self):
-        ms = (ctypes.c_double * 5)()
-        cnt = (ctypes.c_int * 5)()
+        ms = (ctypes.c_double * len(PROFILE_CLASSES))()
+        cnt = (ctypes.c_int * len(PROFILE_CLASSES))()
         _check(lib().lbm_profile_read(self.h, ms, cnt))
         return {k: (ms[q], cnt[q]) for q, k in enumerate(PROFILE_CLASSES)}
 

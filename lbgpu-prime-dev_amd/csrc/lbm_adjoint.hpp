@@ -43,8 +43,18 @@ __device__ __forceinline__ uint32_t push_off(const int (&nb)[19], int s, int a, 
   return foff<19, 2>(s, a, idx);
 }
 
+// Waves per SIMD the register allocation aims at (the even step fits three,
+// the odd step with its neighbour ranks two).
+#ifndef LBM_ADJ_WAVES_ODD
+#define LBM_ADJ_WAVES_ODD 2
+#endif
+#ifndef LBM_ADJ_WAVES_EVEN
+#define LBM_ADJ_WAVES_EVEN 3
+#endif
+
 template <bool ODD, bool OFF32>
-__global__ __launch_bounds__(kBlock) void k_adjoint_step(AdjArgs J) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ODD ? LBM_ADJ_WAVES_ODD : LBM_ADJ_WAVES_EVEN)))
+void k_adjoint_step(AdjArgs J) {
   constexpr int Q = 19;
   const int lb = xcd_block(J.A.nblk8);
   const int idx = lb * kBlock + (int)threadIdx.x;
@@ -61,57 +71,91 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_step(AdjArgs J) {
 #pragma unroll
       for (int a = 1; a < Q; a++) nb[a] = r[a - 1];
     }
-    double f[Q], g[Q], lam[Q], mu[Q];
-    gather2<Q, double, double, ODD, OFF32>(A, idx, nb, f, g);
+    // The transposed collision needs, of the adjoint, its weighted moments
+    //   Lr = sum w lam, L1 = sum w lam c, L2 = sum w lam c c^T,
+    //   M0 = sum w mu,  M1 = sum w mu c
+    // and, of the forward populations, their moments and the two sums
+    // sum lam f, sum mu g; lam and mu themselves are needed once more for
+    // the store.  Every per-direction sum of the collision Jacobian is a
+    // combination of these (D3Q19 moments up to second order), so no
+    // per-direction temporaries are kept.
+    double lam[Q], mu[Q];
+    double Lr = 0.0, L1[3] = {0.0, 0.0, 0.0}, L2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // xx yy zz xy xz yz
+    double M0 = 0.0, M1[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int a = 0; a < Q; a++) {
       const pair_t<double> v = *(const pair_t<double> *)elem<OFF32>(J.L, push_off<ODD>(nb, 0, a, idx));
       lam[a] = v.x;
       mu[a] = v.y;
+      const double wl = W<19>(a) * v.x, wm = W<19>(a) * v.y;
+      Lr += wl;
+      M0 += wm;
+      if (CX(a) != 0) {
+        L1[0] += CX(a) > 0 ? wl : -wl;
+        M1[0] += CX(a) > 0 ? wm : -wm;
+        L2[0] += wl;
+      }
+      if (CY(a) != 0) {
+        L1[1] += CY(a) > 0 ? wl : -wl;
+        M1[1] += CY(a) > 0 ? wm : -wm;
+        L2[1] += wl;
+      }
+      if (CZ(a) != 0) {
+        L1[2] += CZ(a) > 0 ? wl : -wl;
+        M1[2] += CZ(a) > 0 ? wm : -wm;
+        L2[2] += wl;
+      }
+      if (CX(a) * CY(a) != 0) L2[3] += CX(a) * CY(a) > 0 ? wl : -wl;
+      if (CX(a) * CZ(a) != 0) L2[4] += CX(a) * CZ(a) > 0 ? wl : -wl;
+      if (CY(a) * CZ(a) != 0) L2[5] += CY(a) * CZ(a) > 0 ? wl : -wl;
     }
     const double rhoA = A.rhoA, tau = A.tau, om = A.inv_tau, oms = A.inv_tau_s;
-    double drho = 0.0, u[3] = {0.0, 0.0, 0.0}, c = 0.0;
+    double drho = 0.0, u[3] = {0.0, 0.0, 0.0}, c = 0.0, lf = 0.0, mg = 0.0;
 #pragma unroll
     for (int a = 0; a < Q; a++) {
-      drho += f[a];
-      u[0] += CX(a) * f[a];
-      u[1] += CY(a) * f[a];
-      u[2] += CZ(a) * f[a];
-      c += g[a];
+      const pair_t<double> v = *(const pair_t<double> *)elem<OFF32>(A.F, pull_off<ODD>(nb, 0, a, idx));
+      drho += v.x;
+      if (CX(a) != 0) u[0] += CX(a) > 0 ? v.x : -v.x;
+      if (CY(a) != 0) u[1] += CY(a) > 0 ? v.x : -v.x;
+      if (CZ(a) != 0) u[2] += CZ(a) > 0 ? v.x : -v.x;
+      c += v.y;
+      lf += lam[a] * v.x;
+      mg += mu[a] * v.y;
     }
 #pragma unroll
     for (int d = 0; d < 3; d++) u[d] = u[d] / rhoA + tau * A.g[d];
     const double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
     // solute: g* = (1 - ws) g + ws w c (1 + 3 c.u)
-    double Sg = 0.0, Mu[3] = {0.0, 0.0, 0.0}, dts = 0.0;
+    //   Sg = sum mu w (1 + 3 c.u), dts = sum mu (geq - g)
+    const double Sg = M0 + 3.0 * (u[0] * M1[0] + u[1] * M1[1] + u[2] * M1[2]);
+    const double dts = c * Sg - mg;
     // flow: f* = (1 - w) f + w feq(drho, u)
-    double Lr = 0.0, Lu[3] = {0.0, 0.0, 0.0}, dt = 0.0;
+    //   Lu = sum lam w rhoA (3 c + 9 (c.u) c - 3 u), dt = sum lam (feq - f)
+    double L2u[3];
+    L2u[0] = L2[0] * u[0] + L2[3] * u[1] + L2[4] * u[2];
+    L2u[1] = L2[3] * u[0] + L2[1] * u[1] + L2[5] * u[2];
+    L2u[2] = L2[4] * u[0] + L2[5] * u[1] + L2[2] * u[2];
+    double Lu[3];
 #pragma unroll
-    for (int a = 0; a < Q; a++) {
-      const double w = W<19>(a);
-      const double cu = CX(a) * u[0] + CY(a) * u[1] + CZ(a) * u[2];
-      const double geq = w * c * (1.0 + 3.0 * cu);
-      Sg += mu[a] * w * (1.0 + 3.0 * cu);
-      Mu[0] += mu[a] * w * CX(a);
-      Mu[1] += mu[a] * w * CY(a);
-      Mu[2] += mu[a] * w * CZ(a);
-      dts += mu[a] * (geq - g[a]);
-      const double feq = w * (drho + rhoA * (3.0 * cu + 4.5 * cu * cu - 1.5 * usq));
-      Lr += w * lam[a];
-      const double t9 = lam[a] * w * rhoA;
-      Lu[0] += t9 * (3.0 * CX(a) + 9.0 * cu * CX(a) - 3.0 * u[0]);
-      Lu[1] += t9 * (3.0 * CY(a) + 9.0 * cu * CY(a) - 3.0 * u[1]);
-      Lu[2] += t9 * (3.0 * CZ(a) + 9.0 * cu * CZ(a) - 3.0 * u[2]);
-      dt += lam[a] * (feq - f[a]);
-    }
+    for (int d = 0; d < 3; d++) Lu[d] = rhoA * (3.0 * L1[d] + 9.0 * L2u[d] - 3.0 * u[d] * Lr);
+    const double uL1 = u[0] * L1[0] + u[1] * L1[1] + u[2] * L1[2];
+    const double uL2u = u[0] * L2u[0] + u[1] * L2u[1] + u[2] * L2u[2];
+    const double dt = drho * Lr + rhoA * (3.0 * uL1 + 4.5 * uL2u - 1.5 * usq * Lr) - lf;
     double G[3];
 #pragma unroll
-    for (int d = 0; d < 3; d++) G[d] = om * Lu[d] + 3.0 * oms * c * Mu[d];
+    for (int d = 0; d < 3; d++) G[d] = om * Lu[d] + 3.0 * oms * c * M1[d];
     const double seed = k == J.zf ? J.seed : 0.0;
+    double Gr[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) Gr[d] = G[d] / rhoA;
 #pragma unroll
     for (int b = 0; b < Q; b++) {
-      const double lb_ = (1.0 - om) * lam[b] + om * Lr + (CX(b) * G[0] + CY(b) * G[1] + CZ(b) * G[2]) / rhoA;
-      const double mb = (1.0 - oms) * mu[b] + oms * Sg + seed * CZ(b);
+      double lb_ = (1.0 - om) * lam[b] + om * Lr;
+      if (CX(b) != 0) lb_ += CX(b) > 0 ? Gr[0] : -Gr[0];
+      if (CY(b) != 0) lb_ += CY(b) > 0 ? Gr[1] : -Gr[1];
+      if (CZ(b) != 0) lb_ += CZ(b) > 0 ? Gr[2] : -Gr[2];
+      double mb = (1.0 - oms) * mu[b] + oms * Sg;
+      if (CZ(b) != 0) mb += CZ(b) > 0 ? seed : -seed;
       pair_t<double> v;
       v.x = lb_;
       v.y = mb;

@@ -81,15 +81,21 @@ struct lbm_handle {
   int profile = 0;
   std::vector<ProfPair> pending;
   std::vector<hipEvent_t> pool;
-  double prof_ms[5] = {0, 0, 0, 0, 0};
-  int prof_n[5] = {0, 0, 0, 0, 0};
+  double prof_ms[LBM_PROFILE_CLASSES] = {};
+  int prof_n[LBM_PROFILE_CLASSES] = {};
   // objective run and its adjoint
+  // Buffers stay allocated between calls (the next objective or adjoint
+  // run of the same lattice reuses them) and are released with the lattice.
   struct {
     int valid = 0;
     int t0 = 0, nsteps = 0, window = 0, every = 0;
     std::vector<void *> ckpt;  // state before steps t0+1, t0+every+1, ...
     double *jacc = nullptr;
     double J = 0.0;
+    std::vector<void *> seg;   // adjoint: the states of one replayed segment
+    double *L = nullptr;       // adjoint state
+    double *gacc = nullptr;    // per-block parameter gradients
+    double *cgrad = nullptr;   // inlet-map gradient
   } obj;
 };
 
@@ -196,8 +202,13 @@ void free_objective(lbm_handle *h) {
   for (void *q : h->obj.ckpt)
     if (q) (void)hipFree(q);
   h->obj.ckpt.clear();
-  if (h->obj.jacc) (void)hipFree(h->obj.jacc);
-  h->obj.jacc = nullptr;
+  for (void *q : h->obj.seg)
+    if (q) (void)hipFree(q);
+  h->obj.seg.clear();
+  double *bufs[] = {h->obj.jacc, h->obj.L, h->obj.gacc, h->obj.cgrad};
+  for (double *q : bufs)
+    if (q) (void)hipFree(q);
+  h->obj.jacc = h->obj.L = h->obj.gacc = h->obj.cgrad = nullptr;
   h->obj.valid = 0;
 }
 
@@ -856,7 +867,7 @@ extern "C" int lbm_profile_read(lbm_handle *h, double *ms, int *count) {
     h->pool.push_back(pp.b);
   }
   h->pending.clear();
-  for (int c = 0; c < 5; c++) {
+  for (int c = 0; c < LBM_PROFILE_CLASSES; c++) {
     if (ms) ms[c] = h->prof_ms[c];
     if (count) count[c] = h->prof_n[c];
     h->prof_ms[c] = 0;
@@ -919,17 +930,23 @@ extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int ever
   const int t0 = h->time;
   if (t0 + 1 < h->p.sigma_t_on || t0 + nsteps > h->p.sigma_t_off)
     return lbm_fail("the solute must be active for the whole objective run");
-  free_objective(h);
+  h->obj.valid = 0;
   const size_t sb = state_bytes(h);
-  HIP_OK(hipMalloc(&h->obj.jacc, sizeof(double)));
+  const size_t nck = (size_t)((nsteps + every - 1) / every);
+  while (h->obj.ckpt.size() > nck) {
+    (void)hipFree(h->obj.ckpt.back());
+    h->obj.ckpt.pop_back();
+  }
+  while (h->obj.ckpt.size() < nck) {
+    void *c = nullptr;
+    HIP_OK(hipMalloc(&c, sb));
+    h->obj.ckpt.push_back(c);
+  }
+  if (!h->obj.jacc) HIP_OK(hipMalloc(&h->obj.jacc, sizeof(double)));
   HIP_OK(hipMemsetAsync(h->obj.jacc, 0, sizeof(double), h->stream));
   for (int q = 0; q < nsteps; q++) {
-    if (q % every == 0) {
-      void *c = nullptr;
-      HIP_OK(hipMalloc(&c, sb));
-      h->obj.ckpt.push_back(c);
-      HIP_OK(hipMemcpyAsync(c, h->F, sb, hipMemcpyDeviceToDevice, h->stream));
-    }
+    if (q % every == 0)
+      HIP_OK(hipMemcpyAsync(h->obj.ckpt[q / every], h->F, sb, hipMemcpyDeviceToDevice, h->stream));
     const bool in_window = q >= nsteps - window;
     if (do_step(h, q == nsteps - 1, in_window ? h->obj.jacc : nullptr)) return 1;
   }
@@ -955,8 +972,6 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
   const int nseg = (int)h->obj.ckpt.size();
   const int saved_time = h->time;
   void *const saved_F = h->F;  // the replay points h->F at segment buffers
-  std::vector<void *> seg(K + 1, nullptr);
-  double *L = nullptr, *gacc = nullptr, *cgrad = nullptr;
   const int nblk = 8 * h->nblk8;
   int rc = 0;
   auto fail = [&](const char *what, hipError_t e) {
@@ -964,11 +979,24 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
     return rc;
   };
   hipError_t e;
-  for (int q = 0; q <= K && !rc; q++)
-    if ((e = hipMalloc(&seg[q], sb)) != hipSuccess) fail("segment buffers", e);
-  if (!rc && (e = hipMalloc(&L, sb)) != hipSuccess) fail("adjoint state", e);
-  if (!rc && (e = hipMalloc(&gacc, (size_t)nblk * 5 * sizeof(double))) != hipSuccess) fail("gradients", e);
-  if (!rc && (e = hipMalloc(&cgrad, (size_t)h->E * sizeof(double))) != hipSuccess) fail("inlet gradient", e);
+  // workspace: K + 1 segment states, the adjoint state and the gradient
+  // accumulators, kept for the next call
+  std::vector<void *> &seg = h->obj.seg;
+  while (seg.size() > (size_t)K + 1) {
+    (void)hipFree(seg.back());
+    seg.pop_back();
+  }
+  while (!rc && seg.size() < (size_t)K + 1) {
+    void *q = nullptr;
+    if ((e = hipMalloc(&q, sb)) != hipSuccess) fail("segment buffers", e);
+    else seg.push_back(q);
+  }
+  if (!rc && !h->obj.L && (e = hipMalloc(&h->obj.L, sb)) != hipSuccess) fail("adjoint state", e);
+  if (!rc && !h->obj.gacc && (e = hipMalloc(&h->obj.gacc, (size_t)nblk * 5 * sizeof(double))) != hipSuccess)
+    fail("gradients", e);
+  if (!rc && !h->obj.cgrad && (e = hipMalloc(&h->obj.cgrad, (size_t)h->E * sizeof(double))) != hipSuccess)
+    fail("inlet gradient", e);
+  double *L = h->obj.L, *gacc = h->obj.gacc, *cgrad = h->obj.cgrad;
   if (!rc) {
     (void)hipMemsetAsync(L, 0, sb, h->stream);
     (void)hipMemsetAsync(gacc, 0, (size_t)nblk * 5 * sizeof(double), h->stream);
@@ -995,6 +1023,7 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
       A.Fo = (double *)seg[q];
       A.solute_on = 1;
       dim3 grid(nblk), block(kBlock);
+      Prof pr(h, (t & 1) ? 0 : 1);
       if (t & 1) {
         if (off32) hipLaunchKernelGGL((k_half_step<19, 2, double, double, true, true>), grid, block, 0, h->stream, A);
         else hipLaunchKernelGGL((k_half_step<19, 2, double, double, true, false>), grid, block, 0, h->stream, A);
@@ -1016,6 +1045,7 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
       J.zf = h->p.flux_plane;
       J.gacc = gacc;
       dim3 grid(nblk), block(kBlock);
+      Prof pr(h, 5);
       if (odd) {
         if (off32) hipLaunchKernelGGL((k_adjoint_step<true, true>), grid, block, 0, h->stream, J);
         else hipLaunchKernelGGL((k_adjoint_step<true, false>), grid, block, 0, h->stream, J);
@@ -1068,11 +1098,6 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
     }
   }
   (void)hipStreamSynchronize(h->stream);
-  for (void *q : seg)
-    if (q) (void)hipFree(q);
-  if (L) (void)hipFree(L);
-  if (gacc) (void)hipFree(gacc);
-  if (cgrad) (void)hipFree(cgrad);
   h->time = saved_time;
   return rc;
 }

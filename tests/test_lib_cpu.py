@@ -65,3 +65,11 @@ def test_geometry_rejects_bad_arguments():
     L = _pkg.load()
     with pytest.raises(L.LbmError):
         L.discs2d(16, 16, 1, 9.0, 10.0, 0.7, walls=True)
+
+
+def test_profile_class_count_matches_header():
+    import re as _re
+    L = _pkg.load()
+    src = open(os.path.join(ROOT, "include", "lbm_mi355x.h")).read()
+    n = int(_re.search(r"#define LBM_PROFILE_CLASSES (\d+)", src).group(1))
+    assert n == len(L.PROFILE_CLASSES)
