@@ -23,10 +23,13 @@ namespace lbm {
 
 struct AdjArgs {
   StepArgs<double, double> A;  // A.F: forward state before the step, after its BCs
+                               // unless A.zb.on: then the face nodes' boundary
+                               // values are formed here from their known ones
   double *L;                   // adjoint state, same tile layout as the populations
   double seed;                 // dJ / d(plane flux of this step); 0 outside the window
   int zf;                      // flux plane
   double *gacc;                // [launched blocks][5]: dJ/dg_x, g_y, g_z, tau, tau_s
+  double *cgrad;               // [LY][LX] dJ/dc_bot_map (face-node kernel)
 };
 
 // slot offset a half-step reads for direction a (the pull) / writes (the push)
@@ -43,133 +46,351 @@ __device__ __forceinline__ uint32_t push_off(const int (&nb)[19], int s, int a, 
   return foff<19, 2>(s, a, idx);
 }
 
-// Waves per SIMD the register allocation aims at (the even step fits three,
-// the odd step with its neighbour ranks two).
+// Waves per SIMD the register allocation aims at: the even step fits three;
+// the odd step, with its neighbour ranks, ran faster at one wave (about 280
+// registers) than squeezed to two with spills (measured at 128^3).
 #ifndef LBM_ADJ_WAVES_ODD
-#define LBM_ADJ_WAVES_ODD 2
+#define LBM_ADJ_WAVES_ODD 1
 #endif
 #ifndef LBM_ADJ_WAVES_EVEN
 #define LBM_ADJ_WAVES_EVEN 3
 #endif
 
+// Weighted moments of the adjoint of one node:
+//   Lr = sum w lam, L1 = sum w lam c, L2 = sum w lam c c^T (xx yy zz xy xz yz),
+//   M0 = sum w mu,  M1 = sum w mu c.
+// Every per-direction sum of the transposed collision is a combination of
+// these (D3Q19 moments up to second order), so no per-direction
+// temporaries are needed.
+struct AdjMoments {
+  double Lr = 0.0, L1[3] = {0.0, 0.0, 0.0}, L2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double M0 = 0.0, M1[3] = {0.0, 0.0, 0.0};
+};
+
+// the adjoint at the push positions (where the forward half-step wrote)
 template <bool ODD, bool OFF32>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ODD ? LBM_ADJ_WAVES_ODD : LBM_ADJ_WAVES_EVEN)))
-void k_adjoint_step(AdjArgs J) {
-  constexpr int Q = 19;
-  const int lb = xcd_block(J.A.nblk8);
-  const int idx = lb * kBlock + (int)threadIdx.x;
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  if (idx < J.A.nf) {
-    const StepArgs<double, double> &A = J.A;
-    int nb[Q];
-    nb[0] = idx;
-    int i, j, k;
-    coords<3>(A.G, (uint32_t)A.G.node[idx], i, j, k);
-    if (ODD) {
-      int r[18];
-      neighbors_rows<3>(A.G, i, j, k, r);
+__device__ __forceinline__ void adj_load(const AdjArgs &J, const int (&nb)[19], int idx, double (&lam)[19],
+                                         double (&mu)[19], AdjMoments &m) {
 #pragma unroll
-      for (int a = 1; a < Q; a++) nb[a] = r[a - 1];
+  for (int a = 0; a < 19; a++) {
+    const pair_t<double> v = ld_stream<!ODD>((const pair_t<double> *)elem<OFF32>(J.L, push_off<ODD>(nb, 0, a, idx)));
+    lam[a] = v.x;
+    mu[a] = v.y;
+    const double wl = W<19>(a) * v.x, wm = W<19>(a) * v.y;
+    m.Lr += wl;
+    m.M0 += wm;
+    if (CX(a) != 0) {
+      m.L1[0] += CX(a) > 0 ? wl : -wl;
+      m.M1[0] += CX(a) > 0 ? wm : -wm;
+      m.L2[0] += wl;
     }
-    // The transposed collision needs, of the adjoint, its weighted moments
-    //   Lr = sum w lam, L1 = sum w lam c, L2 = sum w lam c c^T,
-    //   M0 = sum w mu,  M1 = sum w mu c
-    // and, of the forward populations, their moments and the two sums
-    // sum lam f, sum mu g; lam and mu themselves are needed once more for
-    // the store.  Every per-direction sum of the collision Jacobian is a
-    // combination of these (D3Q19 moments up to second order), so no
-    // per-direction temporaries are kept.
-    double lam[Q], mu[Q];
-    double Lr = 0.0, L1[3] = {0.0, 0.0, 0.0}, L2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // xx yy zz xy xz yz
-    double M0 = 0.0, M1[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int a = 0; a < Q; a++) {
-      const pair_t<double> v = *(const pair_t<double> *)elem<OFF32>(J.L, push_off<ODD>(nb, 0, a, idx));
-      lam[a] = v.x;
-      mu[a] = v.y;
-      const double wl = W<19>(a) * v.x, wm = W<19>(a) * v.y;
-      Lr += wl;
-      M0 += wm;
-      if (CX(a) != 0) {
-        L1[0] += CX(a) > 0 ? wl : -wl;
-        M1[0] += CX(a) > 0 ? wm : -wm;
-        L2[0] += wl;
-      }
-      if (CY(a) != 0) {
-        L1[1] += CY(a) > 0 ? wl : -wl;
-        M1[1] += CY(a) > 0 ? wm : -wm;
-        L2[1] += wl;
-      }
-      if (CZ(a) != 0) {
-        L1[2] += CZ(a) > 0 ? wl : -wl;
-        M1[2] += CZ(a) > 0 ? wm : -wm;
-        L2[2] += wl;
-      }
-      if (CX(a) * CY(a) != 0) L2[3] += CX(a) * CY(a) > 0 ? wl : -wl;
-      if (CX(a) * CZ(a) != 0) L2[4] += CX(a) * CZ(a) > 0 ? wl : -wl;
-      if (CY(a) * CZ(a) != 0) L2[5] += CY(a) * CZ(a) > 0 ? wl : -wl;
+    if (CY(a) != 0) {
+      m.L1[1] += CY(a) > 0 ? wl : -wl;
+      m.M1[1] += CY(a) > 0 ? wm : -wm;
+      m.L2[1] += wl;
     }
-    const double rhoA = A.rhoA, tau = A.tau, om = A.inv_tau, oms = A.inv_tau_s;
-    double drho = 0.0, u[3] = {0.0, 0.0, 0.0}, c = 0.0, lf = 0.0, mg = 0.0;
-#pragma unroll
-    for (int a = 0; a < Q; a++) {
-      const pair_t<double> v = *(const pair_t<double> *)elem<OFF32>(A.F, pull_off<ODD>(nb, 0, a, idx));
-      drho += v.x;
-      if (CX(a) != 0) u[0] += CX(a) > 0 ? v.x : -v.x;
-      if (CY(a) != 0) u[1] += CY(a) > 0 ? v.x : -v.x;
-      if (CZ(a) != 0) u[2] += CZ(a) > 0 ? v.x : -v.x;
-      c += v.y;
-      lf += lam[a] * v.x;
-      mg += mu[a] * v.y;
+    if (CZ(a) != 0) {
+      m.L1[2] += CZ(a) > 0 ? wl : -wl;
+      m.M1[2] += CZ(a) > 0 ? wm : -wm;
+      m.L2[2] += wl;
     }
-#pragma unroll
-    for (int d = 0; d < 3; d++) u[d] = u[d] / rhoA + tau * A.g[d];
-    const double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    // solute: g* = (1 - ws) g + ws w c (1 + 3 c.u)
-    //   Sg = sum mu w (1 + 3 c.u), dts = sum mu (geq - g)
-    const double Sg = M0 + 3.0 * (u[0] * M1[0] + u[1] * M1[1] + u[2] * M1[2]);
-    const double dts = c * Sg - mg;
-    // flow: f* = (1 - w) f + w feq(drho, u)
-    //   Lu = sum lam w rhoA (3 c + 9 (c.u) c - 3 u), dt = sum lam (feq - f)
-    double L2u[3];
-    L2u[0] = L2[0] * u[0] + L2[3] * u[1] + L2[4] * u[2];
-    L2u[1] = L2[3] * u[0] + L2[1] * u[1] + L2[5] * u[2];
-    L2u[2] = L2[4] * u[0] + L2[5] * u[1] + L2[2] * u[2];
-    double Lu[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) Lu[d] = rhoA * (3.0 * L1[d] + 9.0 * L2u[d] - 3.0 * u[d] * Lr);
-    const double uL1 = u[0] * L1[0] + u[1] * L1[1] + u[2] * L1[2];
-    const double uL2u = u[0] * L2u[0] + u[1] * L2u[1] + u[2] * L2u[2];
-    const double dt = drho * Lr + rhoA * (3.0 * uL1 + 4.5 * uL2u - 1.5 * usq * Lr) - lf;
-    double G[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) G[d] = om * Lu[d] + 3.0 * oms * c * M1[d];
-    const double seed = k == J.zf ? J.seed : 0.0;
-    double Gr[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) Gr[d] = G[d] / rhoA;
-#pragma unroll
-    for (int b = 0; b < Q; b++) {
-      double lb_ = (1.0 - om) * lam[b] + om * Lr;
-      if (CX(b) != 0) lb_ += CX(b) > 0 ? Gr[0] : -Gr[0];
-      if (CY(b) != 0) lb_ += CY(b) > 0 ? Gr[1] : -Gr[1];
-      if (CZ(b) != 0) lb_ += CZ(b) > 0 ? Gr[2] : -Gr[2];
-      double mb = (1.0 - oms) * mu[b] + oms * Sg;
-      if (CZ(b) != 0) mb += CZ(b) > 0 ? seed : -seed;
-      pair_t<double> v;
-      v.x = lb_;
-      v.y = mb;
-      *(pair_t<double> *)elem<OFF32>(J.L, pull_off<ODD>(nb, 0, b, idx)) = v;
-    }
-#pragma unroll
-    for (int d = 0; d < 3; d++) acc[d] = tau * G[d];
-    acc[3] = -om * om * dt + A.g[0] * G[0] + A.g[1] * G[1] + A.g[2] * G[2];
-    acc[4] = -oms * oms * dts;
+    if (CX(a) * CY(a) != 0) m.L2[3] += CX(a) * CY(a) > 0 ? wl : -wl;
+    if (CX(a) * CZ(a) != 0) m.L2[4] += CX(a) * CZ(a) > 0 ? wl : -wl;
+    if (CY(a) * CZ(a) != 0) m.L2[5] += CY(a) * CZ(a) > 0 ? wl : -wl;
   }
+}
+
+// what the transposed collision needs of the forward populations of one
+// node: their moments and the two sums sum lam f, sum mu g
+struct AdjState {
+  double drho = 0.0, j[3] = {0.0, 0.0, 0.0}, c = 0.0, lf = 0.0, mg = 0.0;
+};
+
+template <int A_>
+__device__ __forceinline__ void adj_state_add(AdjState &st, double f, double g, double lam, double mu) {
+  st.drho += f;
+  if (CX(A_) != 0) st.j[0] += CX(A_) > 0 ? f : -f;
+  if (CY(A_) != 0) st.j[1] += CY(A_) > 0 ? f : -f;
+  if (CZ(A_) != 0) st.j[2] += CZ(A_) > 0 ? f : -f;
+  st.c += g;
+  st.lf += lam * f;
+  st.mg += mu * g;
+}
+
+template <int A_, int END>
+__device__ __forceinline__ void adj_state_arrays(AdjState &st, const double (&f)[19], const double (&g)[19],
+                                                 const double (&lam)[19], const double (&mu)[19]) {
+  if constexpr (A_ < END) {
+    adj_state_add<A_>(st, f[A_], g[A_], lam[A_], mu[A_]);
+    adj_state_arrays<A_ + 1, END>(st, f, g, lam, mu);
+  }
+}
+
+template <int A_, int END, bool ODD, bool OFF32>
+__device__ __forceinline__ void adj_state_stream(AdjState &st, const AdjArgs &J, const int (&nb)[19], int idx,
+                                                 const double (&lam)[19], const double (&mu)[19]) {
+  if constexpr (A_ < END) {
+    const pair_t<double> v =
+        ld_stream<!ODD>((const pair_t<double> *)elem<OFF32>(J.A.F, pull_off<ODD>(nb, 0, A_, idx)));
+    adj_state_add<A_>(st, v.x, v.y, lam[A_], mu[A_]);
+    adj_state_stream<A_ + 1, END, ODD, OFF32>(st, J, nb, idx, lam, mu);
+  }
+}
+
+// Transposed collision at one node.  Flow f* = (1 - w) f + w feq(drho, u),
+// solute g* = (1 - ws) g + ws w c (1 + 3 c.u), u = j/rho_A + tau g.
+// Returns the new adjoint through lb(b) = (1 - w) lam_b + w Lr + c_b.Gr and
+// mb(b) = (1 - ws) mu_b + ws Sg (+ the flux seed), and the parameter terms.
+struct AdjOut {
+  double Lr, Sg, Gr[3], acc[5];
+};
+
+__device__ __forceinline__ AdjOut adj_collide(const AdjArgs &J, const AdjMoments &m, const AdjState &st) {
+  const StepArgs<double, double> &A = J.A;
+  const double rhoA = A.rhoA, tau = A.tau, om = A.inv_tau, oms = A.inv_tau_s;
+  double u[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) u[d] = st.j[d] / rhoA + tau * A.g[d];
+  const double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  // solute: Sg = sum mu w (1 + 3 c.u), dts = sum mu (geq - g)
+  const double Sg = m.M0 + 3.0 * (u[0] * m.M1[0] + u[1] * m.M1[1] + u[2] * m.M1[2]);
+  const double dts = st.c * Sg - st.mg;
+  // flow: Lu = sum lam w rhoA (3 c + 9 (c.u) c - 3 u), dt = sum lam (feq - f)
+  double L2u[3];
+  L2u[0] = m.L2[0] * u[0] + m.L2[3] * u[1] + m.L2[4] * u[2];
+  L2u[1] = m.L2[3] * u[0] + m.L2[1] * u[1] + m.L2[5] * u[2];
+  L2u[2] = m.L2[4] * u[0] + m.L2[5] * u[1] + m.L2[2] * u[2];
+  double Lu[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) Lu[d] = rhoA * (3.0 * m.L1[d] + 9.0 * L2u[d] - 3.0 * u[d] * m.Lr);
+  const double uL1 = u[0] * m.L1[0] + u[1] * m.L1[1] + u[2] * m.L1[2];
+  const double uL2u = u[0] * L2u[0] + u[1] * L2u[1] + u[2] * L2u[2];
+  const double dt = st.drho * m.Lr + rhoA * (3.0 * uL1 + 4.5 * uL2u - 1.5 * usq * m.Lr) - st.lf;
+  AdjOut o;
+  double G[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) G[d] = om * Lu[d] + 3.0 * oms * st.c * m.M1[d];
+#pragma unroll
+  for (int d = 0; d < 3; d++) o.Gr[d] = G[d] / rhoA;
+  o.Lr = m.Lr;
+  o.Sg = Sg;
+#pragma unroll
+  for (int d = 0; d < 3; d++) o.acc[d] = tau * G[d];
+  o.acc[3] = -om * om * dt + A.g[0] * G[0] + A.g[1] * G[1] + A.g[2] * G[2];
+  o.acc[4] = -oms * oms * dts;
+  return o;
+}
+
+template <int B_>
+__device__ __forceinline__ void adj_new(const AdjArgs &J, const AdjOut &o, double seed, double lam, double mu,
+                                        double &lb, double &mb) {
+  const double om = J.A.inv_tau, oms = J.A.inv_tau_s;
+  lb = (1.0 - om) * lam + om * o.Lr;
+  if (CX(B_) != 0) lb += CX(B_) > 0 ? o.Gr[0] : -o.Gr[0];
+  if (CY(B_) != 0) lb += CY(B_) > 0 ? o.Gr[1] : -o.Gr[1];
+  if (CZ(B_) != 0) lb += CZ(B_) > 0 ? o.Gr[2] : -o.Gr[2];
+  mb = (1.0 - oms) * mu + oms * o.Sg;
+  if (CZ(B_) != 0) mb += CZ(B_) > 0 ? seed : -seed;
+}
+
+template <int B_, bool ODD, bool OFF32>
+__device__ __forceinline__ void adj_store_all(const AdjArgs &J, const AdjOut &o, double seed, const int (&nb)[19],
+                                              int idx, const double (&lam)[19], const double (&mu)[19]) {
+  if constexpr (B_ < 19) {
+    double nl, nm;
+    adj_new<B_>(J, o, seed, lam[B_], mu[B_], nl, nm);
+    pair_t<double> v;
+    v.x = nl;
+    v.y = nm;
+    st_stream<!ODD>((pair_t<double> *)elem<OFF32>(J.L, pull_off<ODD>(nb, 0, B_, idx)), v);
+    adj_store_all<B_ + 1, ODD, OFF32>(J, o, seed, nb, idx, lam, mu);
+  }
+}
+
+template <int B_>
+__device__ __forceinline__ void adj_new_all(const AdjArgs &J, const AdjOut &o, double seed, const double (&lam)[19],
+                                            const double (&mu)[19], double (&lb)[19], double (&mb)[19]) {
+  if constexpr (B_ < 19) {
+    adj_new<B_>(J, o, seed, lam[B_], mu[B_], lb[B_], mb[B_]);
+    adj_new_all<B_ + 1>(J, o, seed, lam, mu, lb, mb);
+  }
+}
+
+template <bool ODD>
+__device__ __forceinline__ void adj_neighbours(const StepArgs<double, double> &A, int idx, int (&nb)[19], int &k) {
+  nb[0] = idx;
+  int i, j;
+  coords<3>(A.G, (uint32_t)A.G.node[idx], i, j, k);
+  if (ODD) {
+    int r[18];
+    neighbors_rows<3>(A.G, i, j, k, r);
+#pragma unroll
+    for (int a = 1; a < 19; a++) nb[a] = r[a - 1];
+  }
+}
+
+__device__ __forceinline__ void adj_block_sum(const AdjArgs &J, long slot, double (&acc)[5]) {
   __shared__ double red[5];
   const int kinds[5] = {0, 0, 0, 0, 0};
   block_reduce<5>(acc, kinds, red);
-  if (threadIdx.x < 5) J.gacc[(long)lb * 5 + threadIdx.x] += red[threadIdx.x];
+  if (threadIdx.x < 5) J.gacc[slot * 5 + threadIdx.x] += red[threadIdx.x];
+}
+
+// Adjoint of one half-step at every fluid node, or, with A.zb.on, at every
+// node off the two z faces (k_adjoint_face takes those).
+template <bool ODD, bool OFF32>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ODD ? LBM_ADJ_WAVES_ODD : LBM_ADJ_WAVES_EVEN)))
+void k_adjoint_step(AdjArgs J) {
+  const int lb = xcd_block(J.A.nblk8);
+  const int idx = lb * kBlock + (int)threadIdx.x;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (idx < J.A.nf && !zface_node<19>(J.A, idx)) {
+    int nb[19], k;
+    adj_neighbours<ODD>(J.A, idx, nb, k);
+    double lam[19], mu[19];
+    AdjMoments m;
+    adj_load<ODD, OFF32>(J, nb, idx, lam, mu, m);
+    AdjState st;
+    adj_state_stream<0, 19, ODD, OFF32>(st, J, nb, idx, lam, mu);
+    const AdjOut o = adj_collide(J, m, st);
+    adj_store_all<0, ODD, OFF32>(J, o, k == J.zf ? J.seed : 0.0, nb, idx, lam, mu);
+#pragma unroll
+    for (int q = 0; q < 5; q++) acc[q] = o.acc[q];
+  }
+  adj_block_sum(J, lb, acc);
+}
+
+// transpose of the flow z-face condition on the new adjoint of a face node:
+// the unknowns' adjoint goes to the known populations it was formed from
+__device__ __forceinline__ void adj_zface_flow(bool top, double (&l)[19]) {
+  double lu[5], dl[19];
+#pragma unroll
+  for (int a = 0; a < 19; a++) dl[a] = 0.0;
+  if (top) {
+#pragma unroll
+    for (int q = 0; q < 5; q++) lu[q] = l[kDn[q]];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 5; q++) lu[q] = l[kUp[q]];
+  }
+  double lJ, lNx, lNy;
+  if (!top) {
+    // J = drb - (P + 2M); v9 = f10 + J/3, v11 = f12 + J/6 - Nx, v13 = f14 + J/6 + Nx,
+    // v15 = f16 + J/6 - Ny, v17 = f18 + J/6 + Ny
+    lJ = lu[0] / 3.0 + (lu[1] + lu[2] + lu[3] + lu[4]) / 6.0;
+    lNx = -lu[1] + lu[2];
+    lNy = -lu[3] + lu[4];
+#pragma unroll
+    for (int q = 0; q < 5; q++) dl[kDn[q]] += lu[q] - 2.0 * lJ;
+#pragma unroll
+    for (int a = 0; a < 9; a++) dl[a] -= lJ;
+  } else {
+    // J = (P + 2M) - drb; v10 = f9 - J/3, v12 = f11 - J/6 + Nx, v14 = f13 - J/6 - Nx,
+    // v16 = f15 - J/6 + Ny, v18 = f17 - J/6 - Ny
+    lJ = -lu[0] / 3.0 - (lu[1] + lu[2] + lu[3] + lu[4]) / 6.0;
+    lNx = lu[1] - lu[2];
+    lNy = lu[3] - lu[4];
+#pragma unroll
+    for (int q = 0; q < 5; q++) dl[kUp[q]] += lu[q] + 2.0 * lJ;
+#pragma unroll
+    for (int a = 0; a < 9; a++) dl[a] += lJ;
+  }
+  // Nx = ((f1 + f5 + f8) - (f2 + f7 + f6)) / 2, Ny = ((f3 + f5 + f7) - (f4 + f6 + f8)) / 2
+  dl[1] += lNx / 2.0;
+  dl[5] += lNx / 2.0;
+  dl[8] += lNx / 2.0;
+  dl[2] -= lNx / 2.0;
+  dl[7] -= lNx / 2.0;
+  dl[6] -= lNx / 2.0;
+  dl[3] += lNy / 2.0;
+  dl[5] += lNy / 2.0;
+  dl[7] += lNy / 2.0;
+  dl[4] -= lNy / 2.0;
+  dl[6] -= lNy / 2.0;
+  dl[8] -= lNy / 2.0;
+#pragma unroll
+  for (int a = 0; a < 19; a++) l[a] += dl[a];
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const double od = l[kDn[q]], ou = l[kUp[q]];
+    l[kDn[q]] = top ? 0.0 : od;
+    l[kUp[q]] = top ? ou : 0.0;
+  }
+}
+
+// transpose of the solute z-face condition; returns dJ/dc_b of the node
+__device__ __forceinline__ double adj_zface_solute(bool top, double (&l)[19]) {
+  double lcs = 0.0;
+  if (top) {
+#pragma unroll
+    for (int q = 0; q < 5; q++) lcs += W<19>(kDn[q]) * l[kDn[q]];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 5; q++) lcs += W<19>(kUp[q]) * l[kUp[q]];
+  }
+  const double k6 = lcs / (1. / 6.);
+  // bottom: the known in-plane and down-movers lose k6, the up-movers are
+  // cleared; top: the up-movers gain k6, the down-movers are cleared
+  // (value selects only: the elements stay registers)
+  const double kb = top ? 0.0 : k6;
+#pragma unroll
+  for (int a = 0; a < 9; a++) l[a] -= kb;
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const double od = l[kDn[q]], ou = l[kUp[q]];
+    l[kDn[q]] = top ? 0.0 : od - k6;
+    l[kUp[q]] = top ? ou + k6 : 0.0;
+  }
+  return top ? 0.0 : k6;
+}
+
+// Adjoint of one half-step at the fluid nodes of the two z faces, with the
+// face conditions of the forward step formed from the state (as the fused
+// forward half-step formed them) and their transpose applied before the
+// store.  Thread t < lo_end takes compact node t, the others the top face.
+template <bool ODD, bool OFF32>
+__global__ __launch_bounds__(kBlock) void k_adjoint_face(AdjArgs J, int slot0) {
+  const StepArgs<double, double> &A = J.A;
+  const int t = blockIdx.x * kBlock + (int)threadIdx.x;
+  const int nlo = A.zb.lo_end, nface = nlo + (A.nf - A.zb.hi_begin);
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (t < nface) {
+    const int idx = t < nlo ? t : A.zb.hi_begin + (t - nlo);
+    const bool top = idx >= A.zb.hi_begin;
+    int nb[19], k;
+    adj_neighbours<ODD>(A, idx, nb, k);
+    double lam[19], mu[19];
+    AdjMoments m;
+    adj_load<ODD, OFF32>(J, nb, idx, lam, mu, m);
+    double f[19], g[19];
+#pragma unroll
+    for (int a = 0; a < 19; a++) {
+      const pair_t<double> v = *(const pair_t<double> *)elem<OFF32>(A.F, pull_off<ODD>(nb, 0, a, idx));
+      f[a] = v.x;
+      g[a] = v.y;
+    }
+    zface_fix<double, double, true>(A, idx, f, g);
+    AdjState st;
+    adj_state_arrays<0, 19>(st, f, g, lam, mu);
+    const AdjOut o = adj_collide(J, m, st);
+    double lb[19], mb[19];
+    adj_new_all<0>(J, o, k == J.zf ? J.seed : 0.0, lam, mu, lb, mb);
+    if (A.zb.flow) adj_zface_flow(top, lb);
+    if (A.zb.solute) {
+      const double dcb = adj_zface_solute(top, mb);
+      if (!top && J.cgrad) J.cgrad[A.G.node[idx]] += dcb;
+    }
+#pragma unroll
+    for (int b = 0; b < 19; b++) {
+      pair_t<double> v;
+      v.x = lb[b];
+      v.y = mb[b];
+      *(pair_t<double> *)elem<OFF32>(J.L, pull_off<ODD>(nb, 0, b, idx)) = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) acc[q] = o.acc[q];
+  }
+  adj_block_sum(J, slot0 + (long)blockIdx.x, acc);
 }
 
 // Transpose of the z-face conditions of k_bc_zface for one face.

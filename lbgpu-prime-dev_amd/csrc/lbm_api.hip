@@ -69,6 +69,7 @@ struct lbm_handle {
   std::vector<int> node_host;
   int *face_lo = nullptr, *face_hi = nullptr, *face_flux = nullptr;
   int n_lo = 0, n_hi = 0, n_flux = 0;
+  int zb_ok = 0;  // the z faces are the compact ranges [0, n_lo) and [nf - n_hi, nf)
   int *plane_lo = nullptr, *plane_hi = nullptr, *tgt_lo = nullptr, *tgt_hi = nullptr;
   void *halo_pos = nullptr, *halo_neg = nullptr;
   double *cmap = nullptr;
@@ -175,7 +176,37 @@ StepArgs<ST, CT> make_args(const lbm_handle *h) {
     A.inv_scale[s] = (CT)(1.0 / sc);
   }
   A.c0 = (CT)(h->p.storage == LBM_STORE_F64 ? 0.0 : p.store_shift_c);
+  A.zb = ZfaceFuse{};
   return A;
+}
+
+// Whether the z-face conditions of step t can be formed inside the
+// half-step (k_half_step / k_adjoint_face) instead of a separate launch
+// storing them first.  Not when the plane flux is taken on a face (it reads
+// the stored boundary values) or when the solute condition would write into
+// a resting solute (the separate launch does, and later steps see it).
+bool zface_fused(const lbm_handle *h, int t) {
+  const lbm_params &p = h->p;
+  if (h->D != 3 || !h->zb_ok) return false;
+  const bool sol = p.nsubs == 2 && p.zface_solute;
+  if (!p.zface_pressure && !sol) return false;
+  if (h->n_flux && (p.flux_plane == 0 || p.flux_plane == p.nk - 1)) return false;
+  if (sol && !(t >= p.sigma_t_on && t <= p.sigma_t_off)) return false;
+  return true;
+}
+
+template <typename ST, typename CT>
+void set_zb(const lbm_handle *h, StepArgs<ST, CT> &A) {
+  const lbm_params &p = h->p;
+  A.zb.on = 1;
+  A.zb.lo_end = h->n_lo;
+  A.zb.hi_begin = h->nf - h->n_hi;
+  A.zb.flow = p.zface_pressure;
+  A.zb.solute = p.nsubs == 2 && p.zface_solute;
+  A.zb.rho_bot = p.rho_bot;
+  A.zb.rho_top = p.rho_top;
+  A.zb.c_b = p.c_bot;
+  A.zb.cmap = h->cmap;
 }
 
 // Calls fn.template operator()<Q, NS, ST, CT>() for the handle's kind.
@@ -268,9 +299,10 @@ int upload_ints(int **dst, const std::vector<int> &v) {
   return 0;
 }
 
-int launch_half(lbm_handle *h, int odd, int t, int write_mv) {
+int launch_half(lbm_handle *h, int odd, int t, int write_mv, bool fuse_bc = false) {
   return dispatch(h, [&]<int Q, int NS, typename ST, typename CT>() -> int {
     StepArgs<ST, CT> A = make_args<ST, CT>(h);
+    if (fuse_bc) set_zb(h, A);
     A.write_mv = write_mv;
     A.solute_on = NS == 2 && t >= h->p.sigma_t_on && t <= h->p.sigma_t_off;
     Prof pr(h, odd ? 0 : 1);
@@ -407,14 +439,15 @@ int do_step(lbm_handle *h, int write_mv, double *flux_acc = nullptr) {
     if (halo_pack_unpack(h, phase, 1, h->halo_pos, h->halo_neg)) return 1;
     if (halo_pack_unpack(h, phase, 0, h->halo_pos, h->halo_neg)) return 1;
   }
-  if (launch_bcs(h, phase, t)) return 1;
+  const bool fz = zface_fused(h, t);
+  if (!fz && launch_bcs(h, phase, t)) return 1;
   int sol_on = h->p.nsubs == 2 && t >= h->p.sigma_t_on && t <= h->p.sigma_t_off;
   // the plane flux is taken on steps with a macroscopic write and inside an objective window
   if (h->n_flux && sol_on && (write_mv || flux_acc)) {
     if (launch_flux(h, phase, flux_acc)) return 1;
     h->flux_valid = 1;
   }
-  if (launch_half(h, odd, t, write_mv)) return 1;
+  if (launch_half(h, odd, t, write_mv, fz)) return 1;
   h->time = t;
   return 0;
 }
@@ -546,6 +579,9 @@ extern "C" int lbm_set_solids(lbm_handle *h, const uint8_t *mask) {
   }
   h->n_lo = (int)lo.size();
   h->n_hi = (int)hi.size();
+  h->zb_ok = L >= 2;
+  for (size_t q = 0; q < lo.size() && h->zb_ok; q++) h->zb_ok = lo[q] == (int)q;
+  for (size_t q = 0; q < hi.size() && h->zb_ok; q++) h->zb_ok = hi[q] == h->nf - h->n_hi + (int)q;
   h->n_flux = h->p.nsubs == 2 ? (int)fl.size() : 0;
   if (upload_ints(&h->face_lo, lo) || upload_ints(&h->face_hi, hi) || upload_ints(&h->face_flux, fl)) return 1;
 
@@ -992,14 +1028,18 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
     else seg.push_back(q);
   }
   if (!rc && !h->obj.L && (e = hipMalloc(&h->obj.L, sb)) != hipSuccess) fail("adjoint state", e);
-  if (!rc && !h->obj.gacc && (e = hipMalloc(&h->obj.gacc, (size_t)nblk * 5 * sizeof(double))) != hipSuccess)
+  // gradient partials: one slot per block of the step kernel, then one per
+  // block of the face-node kernel
+  const int nfb = (int)blocks_for((long)h->n_lo + h->n_hi, kBlock);
+  const int nslot = nblk + nfb;
+  if (!rc && !h->obj.gacc && (e = hipMalloc(&h->obj.gacc, (size_t)nslot * 5 * sizeof(double))) != hipSuccess)
     fail("gradients", e);
   if (!rc && !h->obj.cgrad && (e = hipMalloc(&h->obj.cgrad, (size_t)h->E * sizeof(double))) != hipSuccess)
     fail("inlet gradient", e);
   double *L = h->obj.L, *gacc = h->obj.gacc, *cgrad = h->obj.cgrad;
   if (!rc) {
     (void)hipMemsetAsync(L, 0, sb, h->stream);
-    (void)hipMemsetAsync(gacc, 0, (size_t)nblk * 5 * sizeof(double), h->stream);
+    (void)hipMemsetAsync(gacc, 0, (size_t)nslot * 5 * sizeof(double), h->stream);
     (void)hipMemsetAsync(cgrad, 0, (size_t)h->E * sizeof(double), h->stream);
   }
   const double seed = 1.0 / ((double)W_ * (double)h->E);
@@ -1015,11 +1055,13 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
     for (int q = 1; q <= len && !rc; q++) {
       const int t = ts + q;
       h->F = seg[q - 1];
-      if (launch_bcs(h, (t & 1) ? 1 : 2, t)) {
+      const bool fz = zface_fused(h, t);
+      if (!fz && launch_bcs(h, (t & 1) ? 1 : 2, t)) {
         rc = 1;
         break;
       }
       StepArgs<double, double> A = make_args<double, double>(h);
+      if (fz) set_zb(h, A);
       A.Fo = (double *)seg[q];
       A.solute_on = 1;
       dim3 grid(nblk), block(kBlock);
@@ -1039,11 +1081,14 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
       const bool odd = t & 1;
       AdjArgs J;
       h->F = seg[q - 1];
+      const bool fz = zface_fused(h, t);
       J.A = make_args<double, double>(h);
+      if (fz) set_zb(h, J.A);
       J.L = L;
       J.seed = t > t0 + T - W_ ? seed : 0.0;
       J.zf = h->p.flux_plane;
       J.gacc = gacc;
+      J.cgrad = cgrad;
       dim3 grid(nblk), block(kBlock);
       Prof pr(h, 5);
       if (odd) {
@@ -1056,6 +1101,24 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
       if ((e = hipGetLastError()) != hipSuccess) {
         fail("adjoint step", e);
         break;
+      }
+      if (fz) {
+        // the face nodes: adjoint step with the face conditions and their transpose
+        if (nfb) {
+          dim3 fgrid(nfb);
+          if (odd) {
+            if (off32) hipLaunchKernelGGL((k_adjoint_face<true, true>), fgrid, block, 0, h->stream, J, nblk);
+            else hipLaunchKernelGGL((k_adjoint_face<true, false>), fgrid, block, 0, h->stream, J, nblk);
+          } else {
+            if (off32) hipLaunchKernelGGL((k_adjoint_face<false, true>), fgrid, block, 0, h->stream, J, nblk);
+            else hipLaunchKernelGGL((k_adjoint_face<false, false>), fgrid, block, 0, h->stream, J, nblk);
+          }
+          if ((e = hipGetLastError()) != hipSuccess) {
+            fail("adjoint face step", e);
+            break;
+          }
+        }
+        continue;
       }
       const lbm_params &p = h->p;
       if (p.zface_pressure || p.zface_solute) {
@@ -1082,14 +1145,14 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
   }
   h->F = saved_F;
   if (!rc) {
-    std::vector<double> part((size_t)nblk * 5);
+    std::vector<double> part((size_t)nslot * 5);
     if ((e = hipMemcpyAsync(part.data(), gacc, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream)) !=
             hipSuccess ||
         (e = hipStreamSynchronize(h->stream)) != hipSuccess) {
       fail("gradient read-back", e);
     } else {
       for (int q = 0; q < 5; q++) grad[q] = 0.0;
-      for (int b = 0; b < nblk; b++)
+      for (int b = 0; b < nslot; b++)
         for (int q = 0; q < 5; q++) grad[q] += part[(size_t)b * 5 + q];
       if (grad_cmap) {
         if ((e = hipMemcpy(grad_cmap, cgrad, (size_t)h->E * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)

@@ -91,6 +91,18 @@ __device__ __forceinline__ void st_stream(T *p, T v) {
   else *p = v;
 }
 
+// 3-D z-face conditions applied by the half-step itself to the populations
+// a face node has gathered (the values k_bc_zface would have stored where
+// the node reads them).  Compact indices [0, lo_end) are the fluid nodes
+// of z = 0 and [hi_begin, nf) those of z = nk - 1 (dense order, z slowest).
+struct ZfaceFuse {
+  int on;
+  int lo_end, hi_begin;
+  int flow, solute;
+  double rho_bot, rho_top, c_b;
+  const double *cmap;  // [nj][ni] inlet concentration or null
+};
+
 template <typename ST, typename CT>
 struct StepArgs {
   ST *F;          // tiles [nf/64][NS][Q][64]: populations the half-step reads
@@ -105,6 +117,7 @@ struct StepArgs {
   CT rhoA, tau, inv_tau, g[3];
   CT rhoA_s, inv_tau_s, sink_fac;
   CT scale[2], inv_scale[2], c0;
+  ZfaceFuse zb;   // zb.on = 0: no boundary work in the half-step
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs; map them so that
@@ -276,6 +289,109 @@ __device__ __forceinline__ void collide(const StepArgs<ST, CT> &A, int idx, CT (
   }
 }
 
+// ---- 3-D z-face conditions (no reference counterpart) -------------------
+// Directions moving up (+z) and down (-z); the unknown populations of a
+// bottom-face node are the up-movers, of a top-face node the down-movers.
+constexpr int kUp[5] = {9, 11, 13, 15, 17};
+constexpr int kDn[5] = {10, 12, 14, 16, 18};
+
+// Zou-He pressure with zero tangential velocity (Hecht & Harting's D3Q19
+// form): the five unknowns of a face node from its known populations and
+// the prescribed density offset drb = rho_b - rho_A.
+template <typename CT>
+__device__ __forceinline__ void zface_flow_values(const CT (&f)[19], bool top, CT drb, CT (&v)[5]) {
+  CT P = f[0];
+#pragma unroll
+  for (int a = 1; a < 9; a++) P += f[a];
+  // (constant indices in both branches: a select between two elements
+  // would index the array at run time and move it to scratch memory)
+  CT M = (CT)0;
+  if (top) {
+#pragma unroll
+    for (int q = 0; q < 5; q++) M += f[kUp[q]];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 5; q++) M += f[kDn[q]];
+  }
+  const CT Nx = ((f[1] + f[5] + f[8]) - (f[2] + f[7] + f[6])) / (CT)2;
+  const CT Ny = ((f[3] + f[5] + f[7]) - (f[4] + f[6] + f[8])) / (CT)2;
+  if (!top) {
+    const CT J = drb - (P + (CT)2 * M);
+    v[0] = f[10] + J / (CT)3;
+    v[1] = f[12] + J / (CT)6 - Nx;
+    v[2] = f[14] + J / (CT)6 + Nx;
+    v[3] = f[16] + J / (CT)6 - Ny;
+    v[4] = f[18] + J / (CT)6 + Ny;
+  } else {
+    const CT J = (P + (CT)2 * M) - drb;
+    v[0] = f[9] - J / (CT)3;
+    v[1] = f[11] - J / (CT)6 + Nx;
+    v[2] = f[13] - J / (CT)6 - Nx;
+    v[3] = f[15] - J / (CT)6 + Ny;
+    v[4] = f[17] - J / (CT)6 - Ny;
+  }
+}
+
+// Solute: Dirichlet inlet (bottom, c = cb: unknowns w_u c* with c* from the
+// known populations) and zero-gradient outlet (top, c* from the up-movers).
+template <typename CT>
+__device__ __forceinline__ void zface_solute_values(const CT (&g)[19], bool top, CT cb, CT (&v)[5]) {
+  CT known = (CT)0, cs;
+  if (!top) {
+#pragma unroll
+    for (int a = 0; a < 9; a++) known += g[a];
+#pragma unroll
+    for (int q = 0; q < 5; q++) known += g[kDn[q]];
+    cs = (cb - known) / (CT)(1. / 6.);
+#pragma unroll
+    for (int q = 0; q < 5; q++) v[q] = (CT)W<19>(kUp[q]) * cs;
+  } else {
+#pragma unroll
+    for (int q = 0; q < 5; q++) known += g[kUp[q]];
+    cs = known / (CT)(1. / 6.);
+#pragma unroll
+    for (int q = 0; q < 5; q++) v[q] = (CT)W<19>(kDn[q]) * cs;
+  }
+}
+
+// put the five values into the unknown slots of the gathered array, through
+// the storage encoding as a separate boundary launch would
+template <typename ST, typename CT, int SUBS>
+__device__ __forceinline__ void zface_put(const StepArgs<ST, CT> &A, bool top, const CT (&v)[5], CT (&f)[19]) {
+  // value selects only: the elements stay registers
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const CT sd = shift_of<19, ST, CT, SUBS>(A, kDn[q]), su = shift_of<19, ST, CT, SUBS>(A, kUp[q]);
+    const CT vd = Codec<ST, CT>::dec(Codec<ST, CT>::enc(v[q], A.scale[SUBS], sd), A.inv_scale[SUBS], sd);
+    const CT vu = Codec<ST, CT>::dec(Codec<ST, CT>::enc(v[q], A.scale[SUBS], su), A.inv_scale[SUBS], su);
+    const CT od = f[kDn[q]], ou = f[kUp[q]];
+    f[kDn[q]] = top ? vd : od;
+    f[kUp[q]] = top ? ou : vu;
+  }
+}
+
+template <int Q, typename ST, typename CT>
+__device__ __forceinline__ bool zface_node(const StepArgs<ST, CT> &A, int idx) {
+  return Q == 19 && A.zb.on && (idx < A.zb.lo_end || idx >= A.zb.hi_begin);
+}
+
+// the z-face conditions of a face node on its gathered flow (and solute)
+template <typename ST, typename CT, bool SOLUTE>
+__device__ __forceinline__ void zface_fix(const StepArgs<ST, CT> &A, int idx, CT (&f)[19], CT (&g)[19]) {
+  const bool top = idx >= A.zb.hi_begin;
+  CT v[5];
+  if (A.zb.flow) {
+    zface_flow_values<CT>(f, top, (CT)(top ? A.zb.rho_top : A.zb.rho_bot) - A.rhoA, v);
+    zface_put<ST, CT, 0>(A, top, v, f);
+  }
+  if (SOLUTE && A.zb.solute) {
+    CT cb = (CT)A.zb.c_b;
+    if (!top && A.zb.cmap) cb = (CT)A.zb.cmap[A.G.node[idx]];  // z = 0: dense index = i + ni j
+    zface_solute_values<CT>(g, top, cb, v);
+    zface_put<ST, CT, 1>(A, top, v, g);
+  }
+}
+
 template <int Q, int NS, typename ST, typename CT, bool ODD, bool OFF32>
 __global__ __launch_bounds__(kBlock) void k_half_step(StepArgs<ST, CT> A) {
   const int idx = xcd_block(A.nblk8) * kBlock + (int)threadIdx.x;
@@ -287,6 +403,8 @@ __global__ __launch_bounds__(kBlock) void k_half_step(StepArgs<ST, CT> A) {
   if (NS == 2 && A.solute_on) {
     CT g[Q];
     gather2<Q, ST, CT, ODD, OFF32>(A, idx, nb, f, g);
+    if constexpr (Q == 19)
+      if (zface_node<Q>(A, idx)) zface_fix<ST, CT, true>(A, idx, f, g);
     collide<Q, ST, CT, 0, ODD>(A, idx, f, u);
     collide<Q, ST, CT, 1, ODD>(A, idx, g, u);
     scatter2<Q, ST, CT, ODD, OFF32>(A, idx, nb, f, g);
@@ -294,6 +412,8 @@ __global__ __launch_bounds__(kBlock) void k_half_step(StepArgs<ST, CT> A) {
   }
   // flow alone (the solute, when present, is outside its active time window)
   gather<Q, NS, ST, CT, 0, ODD, OFF32>(A, idx, nb, f);
+  if constexpr (Q == 19)
+    if (zface_node<Q>(A, idx)) zface_fix<ST, CT, false>(A, idx, f, f);
   collide<Q, ST, CT, 0, ODD>(A, idx, f, u);
   scatter<Q, NS, ST, CT, 0, ODD, OFF32>(A, idx, nb, f);
 }
@@ -413,56 +533,31 @@ __global__ void k_bc_zface(BcArgs<ST, CT> B, ZfaceArgs Z) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= B.n) return;
   const int idx = B.list[t];
-  constexpr int UP[5] = {9, 11, 13, 15, 17};
-  constexpr int DN[5] = {10, 12, 14, 16, 18};
   int nb[19];
-  CT f[19];
+  CT f[19], v[5];
+  const bool top = Z.top;
   if (Z.flow) {
     bc_gather<19, NS, ST, CT, 0>(B, idx, nb, f);
-    CT P = f[0];
+    zface_flow_values<CT>(f, top, (CT)Z.rho_b - B.A.rhoA, v);
+    if (top) {
 #pragma unroll
-    for (int a = 1; a < 9; a++) P += f[a];
-    CT M = (CT)0;
-#pragma unroll
-    for (int q = 0; q < 5; q++) M += f[Z.top ? UP[q] : DN[q]];
-    CT Nx = ((f[1] + f[5] + f[8]) - (f[2] + f[7] + f[6])) / (CT)2;
-    CT Ny = ((f[3] + f[5] + f[7]) - (f[4] + f[6] + f[8])) / (CT)2;
-    CT drb = (CT)Z.rho_b - B.A.rhoA;
-    if (!Z.top) {
-      CT J = drb - (P + (CT)2 * M);
-      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 9, f[10] + J / (CT)3, true);
-      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 11, f[12] + J / (CT)6 - Nx, true);
-      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 13, f[14] + J / (CT)6 + Nx, true);
-      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 15, f[16] + J / (CT)6 - Ny, true);
-      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 17, f[18] + J / (CT)6 + Ny, true);
+      for (int q = 0; q < 5; q++) bc_store<19, NS, ST, CT, 0>(B, idx, nb, kDn[q], v[q], true);
     } else {
-      CT J = (P + (CT)2 * M) - drb;
-      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 10, f[9] - J / (CT)3, true);
-      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 12, f[11] - J / (CT)6 + Nx, true);
-      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 14, f[13] - J / (CT)6 - Nx, true);
-      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 16, f[15] - J / (CT)6 + Ny, true);
-      bc_store<19, NS, ST, CT, 0>(B, idx, nb, 18, f[17] - J / (CT)6 - Ny, true);
+#pragma unroll
+      for (int q = 0; q < 5; q++) bc_store<19, NS, ST, CT, 0>(B, idx, nb, kUp[q], v[q], true);
     }
   }
   if (NS == 2 && Z.solute) {
     bc_gather<19, NS, ST, CT, 1>(B, idx, nb, f);
-    CT known = (CT)0, cs;
-    if (!Z.top) {
-      CT cb = (CT)Z.c_b;
-      if (Z.cmap) cb = (CT)Z.cmap[B.A.G.node[idx]];  // z = 0: dense index = i + ni j
+    CT cb = (CT)Z.c_b;
+    if (!top && Z.cmap) cb = (CT)Z.cmap[B.A.G.node[idx]];  // z = 0: dense index = i + ni j
+    zface_solute_values<CT>(f, top, cb, v);
+    if (top) {
 #pragma unroll
-      for (int a = 0; a < 9; a++) known += f[a];
-#pragma unroll
-      for (int q = 0; q < 5; q++) known += f[DN[q]];
-      cs = (cb - known) / (CT)(1. / 6.);
-#pragma unroll
-      for (int q = 0; q < 5; q++) bc_store<19, NS, ST, CT, 1>(B, idx, nb, UP[q], (CT)W<19>(UP[q]) * cs, true);
+      for (int q = 0; q < 5; q++) bc_store<19, NS, ST, CT, 1>(B, idx, nb, kDn[q], v[q], true);
     } else {
 #pragma unroll
-      for (int q = 0; q < 5; q++) known += f[UP[q]];
-      cs = known / (CT)(1. / 6.);
-#pragma unroll
-      for (int q = 0; q < 5; q++) bc_store<19, NS, ST, CT, 1>(B, idx, nb, DN[q], (CT)W<19>(DN[q]) * cs, true);
+      for (int q = 0; q < 5; q++) bc_store<19, NS, ST, CT, 1>(B, idx, nb, kUp[q], v[q], true);
     }
   }
 }

@@ -202,6 +202,39 @@ def test_pest_partial_macro_write_and_upload(dims):
             assert np.array_equal(got[:, c], full[c][nodes])
 
 
+@pytest.mark.parametrize("case", ["flux_on_face", "solute_starts_late", "flow_only_faces"])
+def test_3d_zface_boundary_paths(case):
+    """the z-face conditions are formed inside the half-step when they can
+    be; the separate boundary launch remains for a flux plane on a face and
+    for steps where the solute rests (its condition still writes there), and
+    both give the oracle's values bit for bit"""
+    say(f"case 3D z-face path {case}")
+    L = _pkg.load()
+    ni, nj, nk = 14, 12, 20
+    mask, _ = L.spheres3d(ni, nj, nk, 405, 2.5, 5.0, 0.6, zbuf=2)
+    rng = np.random.default_rng(8)
+    kw = dict(dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(0.8, 0.75), zface_pressure=1, rho_bot=1.002,
+              rho_top=0.999, zface_solute=1, c_bot_map=rng.uniform(0.8, 1.2, (nj, ni)), flux_plane=nk - 2)
+    if case == "flux_on_face":
+        kw["flux_plane"] = 0
+    if case == "solute_starts_late":
+        kw.update(sigma_t_on=7, sigma_t_off=25)
+    if case == "flow_only_faces":
+        kw.update(zface_solute=0, gaccel=(0, 0, 1e-5, 0, 0, 0))
+    o, g = build_pair(mask, **kw)
+    f, gg = random_state(rng, 19, mask.size, 2)
+    load_both(o, g, f, gg)
+    for t in range(30):
+        o.step(True)
+        g.step(True)
+        if kw["flux_plane"] >= 0 and (kw.get("sigma_t_on", 0) <= t + 1 <= kw.get("sigma_t_off", 10**9)):
+            fo, fg = o.last_flux(), g.last_flux()
+            assert abs(fg[0] - fo[0]) <= 1e-12 * abs(fo[0]) + 1e-300, (t, fo, fg)
+            assert abs(fg[1] - fo[1]) <= 1e-12 * abs(fo[1]) + 1e-300, (t, fo, fg)
+    _, bitwise = compare(o, g, mask, 2, f"z-face path {case}")
+    assert bitwise
+
+
 class DeviceBuffer:
     """device memory from the HIP runtime the library already loaded"""
 
