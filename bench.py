@@ -258,6 +258,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--variants-only", action="store_true",
+                    help="of the extras, run only the 256^3 flow + solute storage variants")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
@@ -346,9 +348,10 @@ def main():
             ext["fp32_over_fp64_speed"] = round(ext["flow_solute_fp32_shifted"]["mflups"]
                                                 / ext["flow_solute_fp64"]["mflups"], 3)
         result["extra"].update(ext)
-        result["extra"]["adjoint_c4"] = adjoint_timing(L)
-        result["extra"]["precision_c3"] = precision_table(L)
-        result["extra"]["slab_c5"] = slab_c5(L)
+        if not args.variants_only:
+            result["extra"]["adjoint_c4"] = adjoint_timing(L)
+            result["extra"]["precision_c3"] = precision_table(L)
+            result["extra"]["slab_c5"] = slab_c5(L)
 
     if not args.no_cpu:
         n_cpu, el, threads = cpu_baseline(mask, dict(base, nsubs=1), args.cpu_seconds)

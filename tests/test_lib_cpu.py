@@ -73,3 +73,22 @@ def test_profile_class_count_matches_header():
     src = open(os.path.join(ROOT, "include", "lbm_mi355x.h")).read()
     n = int(_re.search(r"#define LBM_PROFILE_CLASSES (\d+)", src).group(1))
     assert n == len(L.PROFILE_CLASSES)
+
+
+@pytest.mark.parametrize("name,args,sha,phi", [
+    ("256^3 flow, seed 202", (256, 256, 256, 202, 6.0, 14.0, 0.40),
+     "58d3c9c5de2e132090d9702f02a17a6e2bba6167678a723a13e2f3f19a259d0e", 0.3999288620487336),
+    ("192^3 flow + solute, seed 303", (192, 192, 192, 303, 6.0, 14.0, 0.45),
+     "a3f40ce3a7fcd4651967e742b88ae9fccfdd7d4d57f027e4f2731882d10a4e84", 0.44973209399532005),
+    ("128^3 adjoint, seed 404", (128, 128, 128, 404, 6.0, 14.0, 0.40),
+     "3c1fc171307dd4657989c624b4d869f5dac9a6e1640a4a983cee306d428ce890", 0.3986236572265625),
+    ("448^3 slab, seed 505", (448, 448, 448, 505, 8.0, 20.0, 0.35),
+     "a1c6f0410c42e6daad980d701bbb5adf040cbcc44965e8422acd9049b093b166", 0.34993869307310993),
+])
+def test_benchmark_geometries_recorded(name, args, sha, phi):
+    """the seeded sphere packs of the measured workloads, as .raw bytes
+    (i fastest), are the recorded ones"""
+    L = _pkg.load()
+    m, p = L.spheres3d(*args, zbuf=4)
+    assert hashlib.sha256(m.tobytes()).hexdigest() == sha, name
+    assert p == phi
