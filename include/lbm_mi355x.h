@@ -185,6 +185,16 @@ int lbm_profile_read(lbm_handle *h, double *ms, int *count);
  * (the solute flux through that plane, pre-collision, after the BCs).
  * D3Q19 flow + solute, FP64, halo_mode 0, sink 0. */
 int lbm_objective_run(lbm_handle *h, int nsteps, int window, int every, double *J);
+/* Observation points of the objective (no reference counterpart; the
+ * reference's PEST output samples the same quantity, src/latman.c:1072-1190):
+ * node[i] (dense index, a fluid node), step[i] (1..nsteps of the next
+ * objective run), target[i] and weight[i] (NULL: all 1) add
+ *   sum_i weight[i] * (c(node[i], step[i]) - target[i])^2
+ * to J, c being the solute concentration the step writes (pre-collision,
+ * after the boundary conditions).  With observations, window may be 0 (no
+ * flux term).  n = 0 removes them.  They stay set for later runs. */
+int lbm_objective_observations(lbm_handle *h, int n, const long *node, const int *step, const double *target,
+                               const double *weight);
 /* Gradient of the last objective by the discrete adjoint, replaying each
  * segment between saved states out of place and sweeping it backwards:
  * grad[0..4] = dJ/d(g_x, g_y, g_z, tau, tau_s) of the flow's body force

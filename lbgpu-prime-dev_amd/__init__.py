@@ -96,6 +96,8 @@ def lib():
             "lbm_profile_read": [vp, dp, ctypes.POINTER(ctypes.c_int)],
             "lbm_copy_bandwidth": [ctypes.c_long, ip, dp],
             "lbm_objective_run": [vp, ip, ip, ip, dp],
+            "lbm_objective_observations": [vp, ip, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_int),
+                                           dp, dp],
             "lbm_adjoint_run": [vp, dp, dp],
             "lbm_geom_discs2d": [ip, ip, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_double, ip, u8p, dp],
@@ -313,8 +315,21 @@ class Lattice:
         _check(lib().lbm_last_flux(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def observations(self, node, step, target, weight=None):
+        """lbm_objective_observations: misfit points of the next objective runs"""
+        node = np.ascontiguousarray(node, dtype=np.int64).reshape(-1)
+        step = np.ascontiguousarray(step, dtype=np.int32).reshape(-1)
+        target = np.ascontiguousarray(target, dtype=np.float64).reshape(-1)
+        w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float64).reshape(-1)
+        if not (node.size == step.size == target.size and (w is None or w.size == node.size)):
+            raise ValueError("observation arrays differ in length")
+        _check(lib().lbm_objective_observations(
+            self.h, node.size, node.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+            step.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(target), _dp(w) if w is not None else None))
+
     def objective(self, nsteps, window, every=100):
-        """J over the last `window` of `nsteps` steps (see lbm_objective_run)"""
+        """J over the last `window` of `nsteps` steps, plus the observation
+        misfit when observations are set (see lbm_objective_run)"""
         J = ctypes.c_double()
         _check(lib().lbm_objective_run(self.h, int(nsteps), int(window), int(every), ctypes.byref(J)))
         return J.value

@@ -30,6 +30,8 @@ struct AdjArgs {
   int zf;                      // flux plane
   double *gacc;                // [launched blocks][5]: dJ/dg_x, g_y, g_z, tau, tau_s
   double *cgrad;               // [LY][LX] dJ/dc_bot_map (face-node kernel)
+  const double *obs;           // [nf] dJ/dc of this step's observations (pre-collision
+                               // solute concentration per node); null on other steps
 };
 
 // slot offset a half-step reads for direction a (the pull) / writes (the push)
@@ -181,9 +183,12 @@ __device__ __forceinline__ AdjOut adj_collide(const AdjArgs &J, const AdjMoments
   return o;
 }
 
+// new adjoint of direction B_ at the pull position; seed: dJ/d(plane flux)
+// at the flux plane, oseed: dJ/dc of an observation at this node (c = sum
+// of the pre-collision solute populations)
 template <int B_>
-__device__ __forceinline__ void adj_new(const AdjArgs &J, const AdjOut &o, double seed, double lam, double mu,
-                                        double &lb, double &mb) {
+__device__ __forceinline__ void adj_new(const AdjArgs &J, const AdjOut &o, double seed, double oseed, double lam,
+                                        double mu, double &lb, double &mb) {
   const double om = J.A.inv_tau, oms = J.A.inv_tau_s;
   lb = (1.0 - om) * lam + om * o.Lr;
   if (CX(B_) != 0) lb += CX(B_) > 0 ? o.Gr[0] : -o.Gr[0];
@@ -191,28 +196,31 @@ __device__ __forceinline__ void adj_new(const AdjArgs &J, const AdjOut &o, doubl
   if (CZ(B_) != 0) lb += CZ(B_) > 0 ? o.Gr[2] : -o.Gr[2];
   mb = (1.0 - oms) * mu + oms * o.Sg;
   if (CZ(B_) != 0) mb += CZ(B_) > 0 ? seed : -seed;
+  mb += oseed;
 }
 
 template <int B_, bool ODD, bool OFF32>
-__device__ __forceinline__ void adj_store_all(const AdjArgs &J, const AdjOut &o, double seed, const int (&nb)[19],
-                                              int idx, const double (&lam)[19], const double (&mu)[19]) {
+__device__ __forceinline__ void adj_store_all(const AdjArgs &J, const AdjOut &o, double seed, double oseed,
+                                              const int (&nb)[19], int idx, const double (&lam)[19],
+                                              const double (&mu)[19]) {
   if constexpr (B_ < 19) {
     double nl, nm;
-    adj_new<B_>(J, o, seed, lam[B_], mu[B_], nl, nm);
+    adj_new<B_>(J, o, seed, oseed, lam[B_], mu[B_], nl, nm);
     pair_t<double> v;
     v.x = nl;
     v.y = nm;
     st_stream<!ODD>((pair_t<double> *)elem<OFF32>(J.L, pull_off<ODD>(nb, 0, B_, idx)), v);
-    adj_store_all<B_ + 1, ODD, OFF32>(J, o, seed, nb, idx, lam, mu);
+    adj_store_all<B_ + 1, ODD, OFF32>(J, o, seed, oseed, nb, idx, lam, mu);
   }
 }
 
 template <int B_>
-__device__ __forceinline__ void adj_new_all(const AdjArgs &J, const AdjOut &o, double seed, const double (&lam)[19],
-                                            const double (&mu)[19], double (&lb)[19], double (&mb)[19]) {
+__device__ __forceinline__ void adj_new_all(const AdjArgs &J, const AdjOut &o, double seed, double oseed,
+                                            const double (&lam)[19], const double (&mu)[19], double (&lb)[19],
+                                            double (&mb)[19]) {
   if constexpr (B_ < 19) {
-    adj_new<B_>(J, o, seed, lam[B_], mu[B_], lb[B_], mb[B_]);
-    adj_new_all<B_ + 1>(J, o, seed, lam, mu, lb, mb);
+    adj_new<B_>(J, o, seed, oseed, lam[B_], mu[B_], lb[B_], mb[B_]);
+    adj_new_all<B_ + 1>(J, o, seed, oseed, lam, mu, lb, mb);
   }
 }
 
@@ -253,7 +261,7 @@ void k_adjoint_step(AdjArgs J) {
     AdjState st;
     adj_state_stream<0, 19, ODD, OFF32>(st, J, nb, idx, lam, mu);
     const AdjOut o = adj_collide(J, m, st);
-    adj_store_all<0, ODD, OFF32>(J, o, k == J.zf ? J.seed : 0.0, nb, idx, lam, mu);
+    adj_store_all<0, ODD, OFF32>(J, o, k == J.zf ? J.seed : 0.0, J.obs ? J.obs[idx] : 0.0, nb, idx, lam, mu);
 #pragma unroll
     for (int q = 0; q < 5; q++) acc[q] = o.acc[q];
   }
@@ -374,7 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_face(AdjArgs J, int slot0) {
     adj_state_arrays<0, 19>(st, f, g, lam, mu);
     const AdjOut o = adj_collide(J, m, st);
     double lb[19], mb[19];
-    adj_new_all<0>(J, o, k == J.zf ? J.seed : 0.0, lam, mu, lb, mb);
+    adj_new_all<0>(J, o, k == J.zf ? J.seed : 0.0, J.obs ? J.obs[idx] : 0.0, lam, mu, lb, mb);
     if (A.zb.flow) adj_zface_flow(top, lb);
     if (A.zb.solute) {
       const double dcb = adj_zface_solute(top, mb);
@@ -501,6 +509,19 @@ __global__ void k_adjoint_zface(AdjBcArgs B, ZfaceArgs Z, double *cgrad) {
     if (Z.solute) adj_face<false>(B.L, nb, 1, idx, Z.top, false, dcb);
   }
   if (!Z.top && Z.solute && cgrad) cgrad[B.A.G.node[idx]] += dcb;
+}
+
+// Observation points of the objective (compact node of each; the
+// concentration is the solute mv value a PEST-mode step wrote).
+__global__ void k_obs_gather(const double *mv_c, const int *node, int n, double *out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) out[q] = mv_c[node[q]];
+}
+
+// put (or, with val = null, clear) this step's dJ/dc at its observation nodes
+__global__ void k_obs_seed(double *seed, const int *node, const double *val, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) seed[node[q]] = val ? val[q] : 0.0;
 }
 
 }  // namespace lbm

@@ -97,6 +97,16 @@ struct lbm_handle {
     double *L = nullptr;       // adjoint state
     double *gacc = nullptr;    // per-block parameter gradients
     double *cgrad = nullptr;   // inlet-map gradient
+    // observation points (lbm_objective_observations), sorted by step
+    std::vector<int> o_step, o_node;          // step of the run (1..nsteps), compact node
+    std::vector<double> o_target, o_weight;
+    int *o_node_d = nullptr;                  // compact nodes, sorted by step
+    double *o_c = nullptr;                    // observed concentrations of the last run
+    // dJ/dc per (step, node), merged
+    std::vector<int> s_step, s_begin;         // per step with observations: first entry
+    int *s_node_d = nullptr;
+    double *s_val_d = nullptr;
+    double *seed = nullptr;                   // [nf] per-node dJ/dc of the step being swept
   } obj;
 };
 
@@ -236,10 +246,18 @@ void free_objective(lbm_handle *h) {
   for (void *q : h->obj.seg)
     if (q) (void)hipFree(q);
   h->obj.seg.clear();
-  double *bufs[] = {h->obj.jacc, h->obj.L, h->obj.gacc, h->obj.cgrad};
-  for (double *q : bufs)
+  void *bufs[] = {h->obj.jacc, h->obj.L, h->obj.gacc, h->obj.cgrad, h->obj.o_node_d, h->obj.o_c,
+                  h->obj.s_node_d, h->obj.s_val_d, h->obj.seed};
+  for (void *q : bufs)
     if (q) (void)hipFree(q);
-  h->obj.jacc = h->obj.L = h->obj.gacc = h->obj.cgrad = nullptr;
+  h->obj.jacc = h->obj.L = h->obj.gacc = h->obj.cgrad = h->obj.o_c = h->obj.s_val_d = h->obj.seed = nullptr;
+  h->obj.o_node_d = h->obj.s_node_d = nullptr;
+  h->obj.o_step.clear();
+  h->obj.o_node.clear();
+  h->obj.o_target.clear();
+  h->obj.o_weight.clear();
+  h->obj.s_step.clear();
+  h->obj.s_begin.clear();
   h->obj.valid = 0;
 }
 
@@ -960,9 +978,49 @@ static size_t state_bytes(const lbm_handle *h) {
   return (size_t)h->tiles * h->p.nsubs * h->Q * kTile * h->st_bytes;
 }
 
+extern "C" int lbm_objective_observations(lbm_handle *h, int n, const long *node, const int *step,
+                                          const double *target, const double *weight) {
+  if (need_lattice(h)) return 1;
+  if (n < 0 || (n && (!node || !step || !target))) return lbm_fail("bad observation arguments");
+  std::vector<int> ord(n);
+  for (int q = 0; q < n; q++) {
+    if (node[q] < 0 || node[q] >= h->N || h->fid[node[q]] < 0)
+      return lbm_fail("observation %d: node %ld is not a fluid node of the lattice", q, node[q]);
+    if (step[q] < 1) return lbm_fail("observation %d: step %d (steps count from 1)", q, step[q]);
+    if (weight && !(weight[q] >= 0.0)) return lbm_fail("observation %d: weight must be >= 0", q);
+    ord[q] = q;
+  }
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return step[a] < step[b]; });
+  auto &o = h->obj;
+  o.valid = 0;
+  o.o_step.assign(n, 0);
+  o.o_node.assign(n, 0);
+  o.o_target.assign(n, 0.0);
+  o.o_weight.assign(n, 1.0);
+  for (int q = 0; q < n; q++) {
+    o.o_step[q] = step[ord[q]];
+    o.o_node[q] = h->fid[node[ord[q]]];
+    o.o_target[q] = target[ord[q]];
+    if (weight) o.o_weight[q] = weight[ord[q]];
+  }
+  for (void *p : {(void *)o.o_node_d, (void *)o.o_c}) if (p) (void)hipFree(p);
+  o.o_node_d = nullptr;
+  o.o_c = nullptr;
+  if (n) {
+    HIP_OK(hipMalloc(&o.o_node_d, (size_t)n * sizeof(int)));
+    HIP_OK(hipMemcpy(o.o_node_d, o.o_node.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+    HIP_OK(hipMalloc(&o.o_c, (size_t)n * sizeof(double)));
+  }
+  return 0;
+}
+
 extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int every, double *J) {
   if (need_lattice(h) || adjoint_supported(h)) return 1;
-  if (nsteps < 1 || window < 1 || window > nsteps || every < 1 || !J) return lbm_fail("bad objective arguments");
+  const int nobs = (int)h->obj.o_step.size();
+  if (nsteps < 1 || window < 0 || (window == 0 && nobs == 0) || window > nsteps || every < 1 || !J)
+    return lbm_fail("bad objective arguments");
+  if (nobs && h->obj.o_step.back() > nsteps)
+    return lbm_fail("an observation lies after the run's last step (%d > %d)", h->obj.o_step.back(), nsteps);
   const int t0 = h->time;
   if (t0 + 1 < h->p.sigma_t_on || t0 + nsteps > h->p.sigma_t_off)
     return lbm_fail("the solute must be active for the whole objective run");
@@ -980,22 +1038,72 @@ extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int ever
   }
   if (!h->obj.jacc) HIP_OK(hipMalloc(&h->obj.jacc, sizeof(double)));
   HIP_OK(hipMemsetAsync(h->obj.jacc, 0, sizeof(double), h->stream));
+  auto &o = h->obj;
+  const double *mv_c = h->mv + (long)1 * (1 + h->D) * h->nf;  // solute concentration
+  int ob = 0;
   for (int q = 0; q < nsteps; q++) {
     if (q % every == 0)
-      HIP_OK(hipMemcpyAsync(h->obj.ckpt[q / every], h->F, sb, hipMemcpyDeviceToDevice, h->stream));
+      HIP_OK(hipMemcpyAsync(o.ckpt[q / every], h->F, sb, hipMemcpyDeviceToDevice, h->stream));
     const bool in_window = q >= nsteps - window;
-    if (do_step(h, q == nsteps - 1, in_window ? h->obj.jacc : nullptr)) return 1;
+    int oe = ob;
+    while (oe < nobs && o.o_step[oe] == q + 1) oe++;
+    // an observed step writes the solute concentration (the PEST switch)
+    const int macro = q == nsteps - 1 ? LBM_MACRO_FRAME : oe > ob ? LBM_MACRO_PEST : LBM_MACRO_NONE;
+    if (do_step(h, macro, in_window ? o.jacc : nullptr)) return 1;
+    if (oe > ob) {
+      hipLaunchKernelGGL(k_obs_gather, dim3(blocks_for(oe - ob)), dim3(256), 0, h->stream, mv_c,
+                         o.o_node_d + ob, oe - ob, o.o_c + ob);
+      HIP_OK(hipGetLastError());
+    }
+    ob = oe;
   }
   double acc = 0.0;
-  HIP_OK(hipMemcpyAsync(&acc, h->obj.jacc, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipMemcpyAsync(&acc, o.jacc, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  std::vector<double> c(nobs);
+  if (nobs) HIP_OK(hipMemcpyAsync(c.data(), o.o_c, (size_t)nobs * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipStreamSynchronize(h->stream));
-  h->obj.t0 = t0;
-  h->obj.nsteps = nsteps;
-  h->obj.window = window;
-  h->obj.every = every;
-  h->obj.J = acc / ((double)window * (double)h->E);
-  h->obj.valid = 1;
-  *J = h->obj.J;
+  // observation misfit and its dJ/dc, merged per (step, node)
+  double Jobs = 0.0;
+  std::vector<int> sn;
+  std::vector<double> sv;
+  o.s_step.clear();
+  o.s_begin.clear();
+  for (int q = 0; q < nobs; q++) {
+    const double r = c[q] - o.o_target[q];
+    Jobs += o.o_weight[q] * r * r;
+    const double d = 2.0 * o.o_weight[q] * r;
+    if (o.s_step.empty() || o.s_step.back() != o.o_step[q]) {
+      o.s_step.push_back(o.o_step[q]);
+      o.s_begin.push_back((int)sn.size());
+    }
+    bool merged = false;
+    for (size_t m = o.s_begin.back(); m < sn.size(); m++)
+      if (sn[m] == o.o_node[q]) {
+        sv[m] += d;
+        merged = true;
+      }
+    if (!merged) {
+      sn.push_back(o.o_node[q]);
+      sv.push_back(d);
+    }
+  }
+  o.s_begin.push_back((int)sn.size());
+  for (void *p : {(void *)o.s_node_d, (void *)o.s_val_d}) if (p) (void)hipFree(p);
+  o.s_node_d = nullptr;
+  o.s_val_d = nullptr;
+  if (!sn.empty()) {
+    HIP_OK(hipMalloc(&o.s_node_d, sn.size() * sizeof(int)));
+    HIP_OK(hipMalloc(&o.s_val_d, sv.size() * sizeof(double)));
+    HIP_OK(hipMemcpy(o.s_node_d, sn.data(), sn.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(o.s_val_d, sv.data(), sv.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  o.t0 = t0;
+  o.nsteps = nsteps;
+  o.window = window;
+  o.every = every;
+  o.J = (window > 0 ? acc / ((double)window * (double)h->E) : 0.0) + Jobs;
+  o.valid = 1;
+  *J = o.J;
   return 0;
 }
 
@@ -1036,7 +1144,12 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
     fail("gradients", e);
   if (!rc && !h->obj.cgrad && (e = hipMalloc(&h->obj.cgrad, (size_t)h->E * sizeof(double))) != hipSuccess)
     fail("inlet gradient", e);
+  if (!rc && !h->obj.s_step.empty() && !h->obj.seed) {
+    if ((e = hipMalloc(&h->obj.seed, (size_t)h->nf * sizeof(double))) != hipSuccess) fail("observation seeds", e);
+    else (void)hipMemsetAsync(h->obj.seed, 0, (size_t)h->nf * sizeof(double), h->stream);
+  }
   double *L = h->obj.L, *gacc = h->obj.gacc, *cgrad = h->obj.cgrad;
+  int sidx = (int)h->obj.s_step.size() - 1;  // the sweep visits observed steps backwards
   if (!rc) {
     (void)hipMemsetAsync(L, 0, sb, h->stream);
     (void)hipMemsetAsync(gacc, 0, (size_t)nslot * 5 * sizeof(double), h->stream);
@@ -1089,6 +1202,26 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
       J.zf = h->p.flux_plane;
       J.gacc = gacc;
       J.cgrad = cgrad;
+      J.obs = nullptr;
+      int s_lo = 0, s_n = 0;
+      while (sidx >= 0 && h->obj.s_step[sidx] > t - t0) sidx--;
+      if (sidx >= 0 && h->obj.s_step[sidx] == t - t0) {
+        s_lo = h->obj.s_begin[sidx];
+        s_n = h->obj.s_begin[sidx + 1] - s_lo;
+        hipLaunchKernelGGL(k_obs_seed, dim3(blocks_for(s_n)), dim3(256), 0, h->stream, h->obj.seed,
+                           h->obj.s_node_d + s_lo, h->obj.s_val_d + s_lo, s_n);
+        J.obs = h->obj.seed;
+      }
+      // clears the seeds of this step once its adjoint kernels are queued
+      struct SeedClear {
+        lbm_handle *h;
+        int lo, n;
+        ~SeedClear() {
+          if (n)
+            hipLaunchKernelGGL(k_obs_seed, dim3(blocks_for(n)), dim3(256), 0, h->stream, h->obj.seed,
+                               h->obj.s_node_d + lo, (const double *)nullptr, n);
+        }
+      } seed_clear{h, s_lo, s_n};
       dim3 grid(nblk), block(kBlock);
       Prof pr(h, 5);
       if (odd) {

@@ -52,6 +52,11 @@ struct Setup {
   int ni, nj, nk, nsteps, window, zf;
   const uint8_t *mask;
   double rho_bot, rho_top;
+  // observation points: J += w (c(node, step) - target)^2
+  int nobs;
+  const long *onode;
+  const int *ostep;
+  const double *otarget, *oweight;
 };
 
 template <typename T>
@@ -67,7 +72,7 @@ T run(const Setup &S, T tau, T taus, const T *g, const std::vector<T> &cmap) {
     return (long)i + (long)ni * j + E * k;
   };
   const T one(1.0), om = one / tau, oms = one / taus;
-  T flux(0.0);
+  T flux(0.0), obs(0.0);
   for (int t = 1; t <= S.nsteps; t++) {
     // stream with halfway bounce-back
     for (int k = 0; k < nk; k++)
@@ -125,8 +130,17 @@ T run(const Setup &S, T tau, T taus, const T *g, const std::vector<T> &cmap) {
           }
         }
     }
+    // observations: the pre-collision concentration at (node, step)
+    for (int q = 0; q < S.nobs; q++) {
+      if (S.ostep[q] != t) continue;
+      const long n = S.onode[q];
+      T c = H[n];
+      for (int a = 1; a < 19; a++) c += H[a * N + n];
+      const T r = c - T(S.otarget[q]);
+      obs += T(S.oweight ? S.oweight[q] : 1.0) * r * r;
+    }
     // objective: solute flux through plane zf over the last `window` steps
-    if (t > S.nsteps - S.window) {
+    if (S.window > 0 && t > S.nsteps - S.window) {
       for (int j = 0; j < nj; j++)
         for (int i = 0; i < ni; i++) {
           long n = idx(i, j, S.zf);
@@ -159,7 +173,8 @@ T run(const Setup &S, T tau, T taus, const T *g, const std::vector<T> &cmap) {
       }
     }
   }
-  return flux / T((double)S.window * (double)E);
+  if (S.window > 0) return flux / T((double)S.window * (double)E) + obs;
+  return obs;
 }
 
 }  // namespace
@@ -169,12 +184,16 @@ extern "C" {
 // J and the directional derivative dJ = grad J . (dg, dtau, dtaus, dcmap)
 // for rho_A = 1, started from rest with zero solute.  params[6] =
 // {gx, gy, gz, tau, tau_s, unused}; dparams[5] = {dgx, dgy, dgz, dtau,
-// dtau_s}; cmap, dcmap: [nj][ni].
-int orc_tlm_objective(int ni, int nj, int nk, const uint8_t *mask, double rho_bot, double rho_top, int zf,
-                      int nsteps, int window, const double *params, const double *cmap, const double *dparams,
-                      const double *dcmap, double *J, double *dJ) {
-  if (ni <= 0 || nj <= 0 || nk <= 2 || !mask || !params || !cmap || window < 1 || window > nsteps) return 1;
-  Setup S{ni, nj, nk, nsteps, window, zf, mask, rho_bot, rho_top};
+// dtau_s}; cmap, dcmap: [nj][ni].  J is the flux objective over the last
+// `window` steps (none for window 0) plus the misfit of the nobs
+// observation points (dense node, step 1..nsteps, target, weight or NULL).
+int orc_tlm_objective_obs(int ni, int nj, int nk, const uint8_t *mask, double rho_bot, double rho_top, int zf,
+                          int nsteps, int window, const double *params, const double *cmap, int nobs,
+                          const long *onode, const int *ostep, const double *otarget, const double *oweight,
+                          const double *dparams, const double *dcmap, double *J, double *dJ) {
+  if (ni <= 0 || nj <= 0 || nk <= 2 || !mask || !params || !cmap || window < 0 || window > nsteps) return 1;
+  if (nobs < 0 || (window == 0 && nobs == 0) || (nobs && (!onode || !ostep || !otarget))) return 1;
+  Setup S{ni, nj, nk, nsteps, window, zf, mask, rho_bot, rho_top, nobs, onode, ostep, otarget, oweight};
   const long E = (long)ni * nj;
   if (!dparams) {
     std::vector<double> cm(cmap, cmap + E);
@@ -190,5 +209,14 @@ int orc_tlm_objective(int ni, int nj, int nk, const uint8_t *mask, double rho_bo
   *J = val(r);
   if (dJ) *dJ = der(r);
   return 0;
+}
+
+// the flux objective alone
+int orc_tlm_objective(int ni, int nj, int nk, const uint8_t *mask, double rho_bot, double rho_top, int zf,
+                      int nsteps, int window, const double *params, const double *cmap, const double *dparams,
+                      const double *dcmap, double *J, double *dJ) {
+  if (window < 1) return 1;
+  return orc_tlm_objective_obs(ni, nj, nk, mask, rho_bot, rho_top, zf, nsteps, window, params, cmap, 0, nullptr,
+                               nullptr, nullptr, nullptr, dparams, dcmap, J, dJ);
 }
 }

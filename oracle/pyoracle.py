@@ -78,6 +78,12 @@ def lib():
         L.orc_tlm_objective.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint8),
                                         ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         dp, dp, dp, dp, dp, dp]
+        L.orc_tlm_objective_obs.restype = ctypes.c_int
+        L.orc_tlm_objective_obs.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_uint8), ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_int), dp, dp,
+                                            dp, dp, dp, dp]
         L.orc_num_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -257,9 +263,11 @@ def num_threads():
     return lib().orc_num_threads()
 
 
-def tlm_objective(mask, rho_bot, rho_top, zf, nsteps, window, params, cmap, dparams=None, dcmap=None):
-    """(J, dJ) of the solute-flux objective from the FP64 tangent-linear
-    model (oracle/lbm_tlm.cpp); params = (gx, gy, gz, tau, tau_s)."""
+def tlm_objective(mask, rho_bot, rho_top, zf, nsteps, window, params, cmap, dparams=None, dcmap=None,
+                  observations=None):
+    """(J, dJ) of the solute-flux objective (window > 0) plus the misfit of
+    observations = (dense node, step, target, weight or None) from the FP64
+    tangent-linear model (oracle/lbm_tlm.cpp); params = (gx, gy, gz, tau, tau_s)."""
     m = np.ascontiguousarray(mask, dtype=np.uint8)
     nk, nj, ni = m.shape
     p = np.ascontiguousarray(list(params) + [0.0], dtype=np.float64)
@@ -267,9 +275,19 @@ def tlm_objective(mask, rho_bot, rho_top, zf, nsteps, window, params, cmap, dpar
     J, dJ = ctypes.c_double(), ctypes.c_double()
     dpp = np.ascontiguousarray(dparams, dtype=np.float64) if dparams is not None else None
     dcc = np.ascontiguousarray(dcmap, dtype=np.float64).ravel() if dcmap is not None else None
-    rc = lib().orc_tlm_objective(ni, nj, nk, m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), rho_bot, rho_top,
-                                 zf, nsteps, window, _dp(p), _dp(c), _dp(dpp) if dpp is not None else None,
-                                 _dp(dcc) if dcc is not None else None, ctypes.byref(J), ctypes.byref(dJ))
+    if observations is None:
+        on, os_, ot, ow = (np.zeros(0, np.int64), np.zeros(0, np.int32), np.zeros(0), None)
+    else:
+        on = np.ascontiguousarray(observations[0], dtype=np.int64)
+        os_ = np.ascontiguousarray(observations[1], dtype=np.int32)
+        ot = np.ascontiguousarray(observations[2], dtype=np.float64)
+        ow = None if observations[3] is None else np.ascontiguousarray(observations[3], dtype=np.float64)
+    rc = lib().orc_tlm_objective_obs(
+        ni, nj, nk, m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), rho_bot, rho_top, zf, nsteps, window,
+        _dp(p), _dp(c), on.size, on.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+        os_.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(ot), _dp(ow) if ow is not None else None,
+        _dp(dpp) if dpp is not None else None, _dp(dcc) if dcc is not None else None, ctypes.byref(J),
+        ctypes.byref(dJ))
     if rc:
-        raise ValueError("orc_tlm_objective: bad arguments")
+        raise ValueError("orc_tlm_objective_obs: bad arguments")
     return J.value, dJ.value

@@ -86,6 +86,83 @@ def test_adjoint_matches_central_differences():
     assert abs(gmap[j, i] - fd) <= 1e-5 * abs(fd)
 
 
+def observations(mask):
+    """points in the interior, on both z faces, and one repeated at the same
+    step (its two terms add); steps odd and even, inside and outside the
+    flux window, in several checkpoint segments"""
+    nk, nj, ni = mask.shape
+    rng = np.random.default_rng(33)
+    fl = mask.reshape(-1) != 1
+    flat = np.arange(mask.size)
+    k = flat // (ni * nj)
+    interior = flat[fl & (k > 1) & (k < nk - 2)]
+    bottom = flat[fl & (k == 0)]
+    top = flat[fl & (k == nk - 1)]
+    nodes = np.concatenate([rng.choice(interior, 5, replace=False), bottom[:1], top[:1], interior[:1]])
+    steps = np.array([3, 14, 27, 27, 46, 9, 51, 3])
+    target = rng.uniform(0.0, 0.4, nodes.size)
+    weight = rng.uniform(0.5, 2.0, nodes.size)
+    nodes[-1] = nodes[0]  # a second observation of the first point, same step
+    return nodes, steps, target, weight
+
+
+@pytest.mark.parametrize("window", [0, 20])
+def test_observation_adjoint_matches_tangent_linear_model(window):
+    """observation misfit (the PEST points as a calibration objective), with
+    and without the flux term"""
+    say(f"case observation adjoint vs TLM, window {window}")
+    L, mask, cmap = setup()
+    zf = mask.shape[0] - 2
+    obs = observations(mask)
+    nk, nj, ni = mask.shape
+    lat = L.Lattice(mask=mask, dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(PARAMS[3], PARAMS[4]),
+                    gaccel=tuple(PARAMS[:3]) + (0.0, 0.0, 0.0), zface_pressure=1, rho_bot=1.001,
+                    rho_top=1.0, zface_solute=1, c_bot_map=cmap, flux_plane=zf)
+    lat.init_rest()
+    lat.observations(*obs)
+    J = lat.objective(60, window, 25)
+    grad, gmap = lat.adjoint()
+    J_tlm, _ = tlm_objective(mask, 1.001, 1.0, zf, 60, window, PARAMS, cmap, observations=obs)
+    say(f"J gpu {J:.15e} tlm {J_tlm:.15e}")
+    assert J_tlm > 0 and abs(J - J_tlm) <= 1e-12 * abs(J_tlm)
+    for q in range(5):
+        dp = np.zeros(5)
+        dp[q] = 1.0
+        _, dJ = tlm_objective(mask, 1.001, 1.0, zf, 60, window, PARAMS, cmap, dp, np.zeros_like(cmap),
+                              observations=obs)
+        say(f"param {q}: adjoint {grad[q]:.12e} tlm {dJ:.12e}")
+        assert abs(grad[q] - dJ) <= 1e-9 * max(abs(dJ), 1e-30) + 1e-14
+    dc = np.random.default_rng(19).uniform(-1, 1, cmap.shape)
+    _, dJ = tlm_objective(mask, 1.001, 1.0, zf, 60, window, PARAMS, cmap, np.zeros(5), dc, observations=obs)
+    adj = float(np.sum(gmap * dc))
+    say(f"inlet map direction: adjoint {adj:.12e} tlm {dJ:.12e}")
+    assert abs(adj - dJ) <= 1e-9 * abs(dJ)
+    # a second sweep of the same run gives the same gradient (the seeds are
+    # cleared after their step)
+    grad2, gmap2 = lat.adjoint()
+    assert np.array_equal(grad, grad2) and np.array_equal(gmap, gmap2)
+
+
+def test_observation_arguments_checked():
+    L, mask, cmap = setup()
+    nk, nj, ni = mask.shape
+    lat = L.Lattice(mask=mask, dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(0.8, 0.75), zface_pressure=1,
+                    zface_solute=1, c_bot_map=cmap, flux_plane=nk - 2)
+    lat.init_rest()
+    solid = int(np.flatnonzero(mask.reshape(-1) == 1)[0])
+    with pytest.raises(L.LbmError, match="fluid node"):
+        lat.observations([solid], [3], [0.1])
+    fluid = int(np.flatnonzero(mask.reshape(-1) != 1)[0])
+    with pytest.raises(L.LbmError, match="steps count from 1"):
+        lat.observations([fluid], [0], [0.1])
+    lat.observations([fluid], [40], [0.1])
+    with pytest.raises(L.LbmError, match="after the run"):
+        lat.objective(30, 10, 10)
+    lat.observations([], [], [])
+    with pytest.raises(L.LbmError, match="bad objective"):
+        lat.objective(30, 0, 10)
+
+
 def test_adjoint_rejects_unsupported_configurations():
     L, mask, cmap = setup()
     nk, nj, ni = mask.shape

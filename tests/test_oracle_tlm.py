@@ -61,3 +61,60 @@ def test_tlm_derivative_equals_central_difference(which):
     Jm, _ = tlm_objective(mask, 1.001, 1.0, zf, 50, 15, params - h * dp, cmap - h * dc)
     fd = (Jp - Jm) / (2 * h)
     assert abs(dJ - fd) <= 1e-6 * abs(fd) + 1e-12, (dJ, fd)
+
+
+def observation_case(mask):
+    """observation points at fluid nodes of a few planes and steps"""
+    rng = np.random.default_rng(21)
+    fluid = np.flatnonzero(mask.reshape(-1) != 1)
+    nodes = rng.choice(fluid, 7, replace=False)
+    steps = np.array([5, 12, 12, 30, 31, 44, 50])
+    target = rng.uniform(0.0, 0.3, 7)
+    weight = rng.uniform(0.5, 2.0, 7)
+    return nodes, steps, target, weight
+
+
+def test_tlm_observation_misfit_equals_oracle():
+    """the misfit term of the TLM equals the AA oracle's PEST-written
+    concentrations at the observed steps"""
+    mask, cmap = case()
+    nk, nj, ni = mask.shape
+    params = (0.0, 0.0, 1e-5, 0.8, 0.75)
+    obs = observation_case(mask)
+    J_tlm, _ = tlm_objective(mask, 1.001, 1.0, nk - 2, 50, 0, params, cmap, observations=obs)
+    o = Oracle(dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(0.8, 0.75), gaccel=(0.0, 0.0, 1e-5, 0, 0, 0),
+               zface_pressure=1, rho_bot=1.001, rho_top=1.0, zface_solute=1, c_bot_map=cmap, flux_plane=nk - 2)
+    o.set_solids(mask)
+    o.init_rest()
+    J = 0.0
+    for t in range(1, 51):
+        here = obs[1] == t
+        o.step(2 if here.any() else 0)
+        if here.any():
+            c = o.mv(1)[0]
+            for q in np.flatnonzero(here):
+                J += obs[3][q] * (c[obs[0][q]] - obs[2][q]) ** 2
+    assert J > 0
+    assert abs(J_tlm - J) <= 1e-12 * J
+
+
+@pytest.mark.parametrize("which", [2, 3, 4, 5])
+def test_tlm_observation_derivative_equals_central_difference(which):
+    mask, cmap = case()
+    params = np.array([0.0, 0.0, 1e-5, 0.8, 0.75])
+    zf = mask.shape[0] - 2
+    obs = observation_case(mask)
+    rng = np.random.default_rng(9)
+    dp = np.zeros(5)
+    dc = np.zeros_like(cmap)
+    if which < 5:
+        dp[which] = 1.0
+    else:
+        dc = rng.uniform(-1, 1, cmap.shape)
+    h = 1e-6
+    _, dJ = tlm_objective(mask, 1.001, 1.0, zf, 50, 10, params, cmap, dp, dc, observations=obs)
+    Jp, _ = tlm_objective(mask, 1.001, 1.0, zf, 50, 10, params + h * dp, cmap + h * dc, observations=obs)
+    Jm, _ = tlm_objective(mask, 1.001, 1.0, zf, 50, 10, params - h * dp, cmap - h * dc, observations=obs)
+    fd = (Jp - Jm) / (2 * h)
+    assert dJ != 0.0
+    assert abs(dJ - fd) <= 1e-6 * abs(dJ)
