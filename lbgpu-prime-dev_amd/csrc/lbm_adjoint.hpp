@@ -48,14 +48,17 @@ __device__ __forceinline__ uint32_t push_off(const int (&nb)[19], int s, int a, 
   return foff<19, 2>(s, a, idx);
 }
 
-// Waves per SIMD the register allocation aims at: the even step fits three;
-// the odd step, with its neighbour ranks, ran faster at one wave (about 280
-// registers) than squeezed to two with spills (measured at 128^3).
+// Waves per SIMD the register allocation aims at.  The even step at two
+// waves (about 236 registers): squeezed to three it fits in 150 registers
+// only by reusing one register quadruple for all 19 stores, and each store
+// then waits for every memory access before it (233 us against 133 us per
+// launch at 128^3).  The odd step, with its neighbour ranks, ran faster at
+// one wave (about 280 registers) than squeezed to two with spills.
 #ifndef LBM_ADJ_WAVES_ODD
 #define LBM_ADJ_WAVES_ODD 1
 #endif
 #ifndef LBM_ADJ_WAVES_EVEN
-#define LBM_ADJ_WAVES_EVEN 3
+#define LBM_ADJ_WAVES_EVEN 2
 #endif
 
 // Weighted moments of the adjoint of one node:
