@@ -50,8 +50,10 @@ def algorithmic_bytes(n_fluid, n_nodes, q, nsubs, s):
     return nsubs * n_fluid * q * 2 * s + n_nodes
 
 
-def run_workload(L, mask, kw, steps, warmup, label):
+def run_workload(L, mask, kw, steps, warmup, label, ns=None):
     lat = L.Lattice(mask=mask, **kw)
+    if ns is not None:
+        lat.set_ns(ns)
     info = lat.info()
     lat.init_rest()
     t0 = time.perf_counter()
@@ -259,7 +261,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variants-only", action="store_true",
-                    help="of the extras, run only the 256^3 flow + solute storage variants")
+                    help="of the extras, run only the 256^3 flow + solute storage and Walsh variants")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
@@ -344,6 +346,23 @@ def main():
                 "even_frac": round(B2 / (e_ms * 1e-3) / HBM_PEAK, 4),
                 "step_frac_wall": round(B2 * steps2 / wall2 / HBM_PEAK, 4),
             }
+        # Walsh partial bounce-back as the reference's Walsh build runs it:
+        # every node computed, the sphere voxels carrying ns = 0.5; the
+        # kernel reads the 8-byte ns of each node besides the populations
+        kw2 = dict(base, nsubs=2, zface_solute=1, c_bot=1.0)
+        steps2 = max(args.steps // 2, 10)
+        info2, wall2, prof2, _ = run_workload(L, np.zeros_like(mask), kw2, steps2, 10, "walsh",
+                                              ns=np.where(mask == 1, 0.5, 0.0))
+        B2 = algorithmic_bytes(info2["fluid"], info2["nodes"], 19, 2, 8) + 8 * info2["fluid"]
+        o_ms = prof2["odd"][0] / max(prof2["odd"][1], 1)
+        e_ms = prof2["even"][0] / max(prof2["even"][1], 1)
+        ext["flow_solute_fp64_walsh_all_nodes"] = {
+            "mlups": round(info2["nodes"] * steps2 / wall2 / 1e6, 1),
+            "ms_per_step": round(wall2 / steps2 * 1e3, 4),
+            "odd_frac": round(B2 / (o_ms * 1e-3) / HBM_PEAK, 4),
+            "even_frac": round(B2 / (e_ms * 1e-3) / HBM_PEAK, 4),
+            "step_frac_wall": round(B2 * steps2 / wall2 / HBM_PEAK, 4),
+        }
         if "flow_solute_fp64" in ext and "flow_solute_fp32_shifted" in ext:
             ext["fp32_over_fp64_speed"] = round(ext["flow_solute_fp32_shifted"]["mflups"]
                                                 / ext["flow_solute_fp64"]["mflups"], 3)

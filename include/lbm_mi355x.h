@@ -82,7 +82,11 @@ enum lbm_reduction {
 
 const char *lbm_last_error(void);
 
-/* construct_lattice: allocation, constants (src/latman.c:91-493) */
+/* construct_lattice: allocation, constants (src/latman.c:91-493).  The
+ * kernels address populations with 32-bit byte offsets up to 4 GiB of
+ * populations and with 64-bit offsets above; the environment variable
+ * LBM_OFFSET64=1, read here, selects the 64-bit variants at every size
+ * (same results, so that path can be checked on small lattices). */
 int lbm_create(const lbm_params *p, lbm_handle **out);
 /* destruct_lattice (src/latman.c:2084-2177) */
 int lbm_destroy(lbm_handle *h);
@@ -93,6 +97,17 @@ int lbm_destroy(lbm_handle *h);
  * compaction and the per-row fluid bitmasks the kernels derive neighbours
  * from; populations become zero. */
 int lbm_set_solids(lbm_handle *h, const uint8_t *mask);
+
+/* Walsh partial bounce-back (the reference's WALSH_NS_ON path, off in its
+ * flags: src/k_stream_collide_stream.c:129-135,363-383, src/lattice.c:
+ * 2936-2939; ns_mem_d from process_ns / read_ns, src/latman.c:48-80,
+ * 700-770): ns [N], a fraction in [0, 1] per node (values at solid nodes are
+ * ignored).  Every fluid node then collides to (1 - ns) BGK + ns times its
+ * opposite pre-collision populations, with momentum and force scaled by
+ * 1 - ns.  The reference's Walsh build computes every node; here nodes whose
+ * mask byte is 1 still take halfway bounce-back, so an all-zero mask gives
+ * the reference's model.  NULL turns it off; lbm_set_solids clears it. */
+int lbm_set_ns(lbm_handle *h, const double *ns);
 
 /* f H2D / D2H per subs (src/lbgpu_prime.cu:317-343, :874-900).  Solid
  * nodes are not stored: download returns 0 there. */

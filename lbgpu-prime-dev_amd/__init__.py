@@ -88,6 +88,7 @@ def lib():
             "lbm_halo_periodic": [vp, ip],
             "lbm_download_macro": [vp, ip, dp, dp, dp, dp],
             "lbm_upload_macro": [vp, ip, dp, dp, dp, dp],
+            "lbm_set_ns": [vp, dp],
             "lbm_sample_macro": [vp, ip, ip, ctypes.POINTER(ctypes.c_long), dp],
             "lbm_reduce": [vp, ip, ip, dp],
             "lbm_last_flux": [vp, dp, dp],
@@ -282,6 +283,16 @@ class Lattice:
         rho, ux, uy, uz = (np.empty(self.N) for _ in range(4))
         _check(lib().lbm_download_macro(self.h, subs, _dp(rho), _dp(ux), _dp(uy), _dp(uz)))
         return rho, ux, uy, uz
+
+    def set_ns(self, ns):
+        """lbm_set_ns: Walsh partial bounce-back fraction per node (None: off)"""
+        if ns is None:
+            _check(lib().lbm_set_ns(self.h, None))
+            return
+        a = np.ascontiguousarray(ns, dtype=np.float64).reshape(-1)
+        if a.size != self.N:
+            raise ValueError("ns holds one value per lattice node")
+        _check(lib().lbm_set_ns(self.h, _dp(a)))
 
     def sample_mv(self, subs, nodes):
         """lbm_sample_macro: [len(nodes), 1 + dims] values of mv at dense nodes"""

@@ -70,6 +70,8 @@ struct lbm_handle {
   int *face_lo = nullptr, *face_hi = nullptr, *face_flux = nullptr;
   int n_lo = 0, n_hi = 0, n_flux = 0;
   int zb_ok = 0;  // the z faces are the compact ranges [0, n_lo) and [nf - n_hi, nf)
+  double *ns = nullptr;  // [nf] Walsh partial bounce-back fraction, or null
+  bool off64 = false;    // 64-bit element offsets at every size (LBM_OFFSET64=1)
   int *plane_lo = nullptr, *plane_hi = nullptr, *tgt_lo = nullptr, *tgt_hi = nullptr;
   void *halo_pos = nullptr, *halo_neg = nullptr;
   double *cmap = nullptr;
@@ -187,6 +189,7 @@ StepArgs<ST, CT> make_args(const lbm_handle *h) {
   }
   A.c0 = (CT)(h->p.storage == LBM_STORE_F64 ? 0.0 : p.store_shift_c);
   A.zb = ZfaceFuse{};
+  A.ns = nullptr;
   return A;
 }
 
@@ -264,7 +267,7 @@ void free_objective(lbm_handle *h) {
 void free_lattice(lbm_handle *h) {
   free_objective(h);
   void *ptrs[] = {h->F, h->rw, h->mv, h->node, h->face_lo, h->face_hi, h->face_flux, h->plane_lo,
-                  h->plane_hi, h->tgt_lo, h->tgt_hi, h->halo_pos, h->halo_neg};
+                  h->plane_hi, h->tgt_lo, h->tgt_hi, h->halo_pos, h->halo_neg, h->ns};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   h->F = nullptr;
@@ -274,6 +277,7 @@ void free_lattice(lbm_handle *h) {
   h->face_lo = h->face_hi = h->face_flux = nullptr;
   h->plane_lo = h->plane_hi = h->tgt_lo = h->tgt_hi = nullptr;
   h->halo_pos = h->halo_neg = nullptr;
+  h->ns = nullptr;
   h->nf = 0;
 }
 
@@ -324,9 +328,20 @@ int launch_half(lbm_handle *h, int odd, int t, int write_mv, bool fuse_bc = fals
     A.write_mv = write_mv;
     A.solute_on = NS == 2 && t >= h->p.sigma_t_on && t <= h->p.sigma_t_off;
     Prof pr(h, odd ? 0 : 1);
-    const bool off32 = (size_t)h->tiles * NS * Q * kTile * sizeof(ST) <= 0xffffffffull;
+    const bool off32 = !h->off64 && (size_t)h->tiles * NS * Q * kTile * sizeof(ST) <= 0xffffffffull;
     dim3 grid(8 * h->nblk8), block(kBlock);
-    if (odd && off32)
+    if (h->ns) {
+      // Walsh partial bounce-back
+      A.ns = h->ns;
+      if (odd && off32)
+        hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, true, true, true>), grid, block, 0, h->stream, A);
+      else if (odd)
+        hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, true, false, true>), grid, block, 0, h->stream, A);
+      else if (off32)
+        hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, false, true, true>), grid, block, 0, h->stream, A);
+      else
+        hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, false, false, true>), grid, block, 0, h->stream, A);
+    } else if (odd && off32)
       hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, true, true>), grid, block, 0, h->stream, A);
     else if (odd)
       hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, true, false>), grid, block, 0, h->stream, A);
@@ -501,6 +516,10 @@ extern "C" int lbm_create(const lbm_params *p, lbm_handle **out) {
   h->N = N;
   h->E = p->dims == 2 ? p->ni : (long)p->ni * p->nj;
   h->st_bytes = p->storage == LBM_STORE_F64 ? 8 : (p->storage == LBM_STORE_F32 ? 4 : 2);
+  // the kernels' 64-bit offset variants normally run only above 4 GiB of
+  // populations; LBM_OFFSET64=1 selects them on any lattice
+  const char *o64 = getenv("LBM_OFFSET64");
+  h->off64 = o64 && o64[0] == '1' && o64[1] == 0;
   for (int s = 0; s < 2; s++) {
     if (p->pressure_n_series[s] && p->pressure_n_series_len[s] > 0)
       h->series[s].assign(p->pressure_n_series[s], p->pressure_n_series[s] + p->pressure_n_series_len[s]);
@@ -674,6 +693,25 @@ static int transfer_f(lbm_handle *h, int subs, double *f1d, int up) {
   }
   (void)hipFree(dstage);
   return rc;
+}
+
+extern "C" int lbm_set_ns(lbm_handle *h, const double *ns) {
+  if (need_lattice(h)) return 1;
+  if (h->ns) {
+    HIP_OK(hipStreamSynchronize(h->stream));
+    (void)hipFree(h->ns);
+    h->ns = nullptr;
+  }
+  if (!ns) return 0;
+  std::vector<double> c((size_t)h->nf);
+  for (int x = 0; x < h->nf; x++) {
+    const double v = ns[h->node_host[x]];
+    if (!(v >= 0.0 && v <= 1.0)) return lbm_fail("ns at node %d is %g, outside [0, 1]", h->node_host[x], v);
+    c[x] = v;
+  }
+  HIP_OK(hipMalloc(&h->ns, c.size() * sizeof(double)));
+  HIP_OK(hipMemcpy(h->ns, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
 }
 
 extern "C" int lbm_upload_f(lbm_handle *h, int subs, const double *f1d) {
@@ -971,6 +1009,7 @@ static int adjoint_supported(const lbm_handle *h) {
   if (p.halo_mode) return lbm_fail("the adjoint needs the periodic (halo_mode = 0) lattice");
   if (p.sink != 0.0) return lbm_fail("the adjoint needs sink = 0");
   if (p.flux_plane < 0 || p.flux_plane >= p.nk) return lbm_fail("the objective needs flux_plane");
+  if (h->ns) return lbm_fail("the adjoint does not cover partial bounce-back (lbm_set_ns)");
   return 0;
 }
 
@@ -1156,7 +1195,7 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
     (void)hipMemsetAsync(cgrad, 0, (size_t)h->E * sizeof(double), h->stream);
   }
   const double seed = 1.0 / ((double)W_ * (double)h->E);
-  const bool off32 = sb <= 0xffffffffull;
+  const bool off32 = !h->off64 && sb <= 0xffffffffull;
   for (int sgi = nseg - 1; sgi >= 0 && !rc; sgi--) {
     const int ts = t0 + sgi * K;                 // state seg[0] is before step ts+1
     const int len = std::min(K, t0 + T - ts);

@@ -5,12 +5,18 @@
 // The lbgpu_prime executable is this function behind main().
 //
 //   lbgpu_prime [name] [--dims 2|3] [--subs 1|2] [--storage f64|f32|f16]
-//               [--compute f64|f32] [--in DIR] [--out DIR]
+//               [--compute f64|f32] [--in DIR] [--out DIR] [--walsh]
 //
 // name selects ./in_<name>/params.in as argv[1] does in construct_lattice
 // (src/latman.c:108-116); solids and series are read from ./in (or --in).
 // --dims / --subs stand for the reference's NUM_DIMENSIONS and
 // NUM_FLUID_COMPONENTS build flags (defaults 2 and 1, src/flags.h:104,108).
+// --walsh stands for a build with WALSH_NS_ON and COMPUTE_ON_SOLIDS
+// (src/flags.h:57-63): the kernels then compute every node without
+// bounce-back, and the black pixels of the 2-D solids image carry the
+// partial bounce-back fraction ns of params.in (process_ns,
+// src/latman.c:48-80, 413-416); the host keeps the image's solids for its
+// frame statistics and .dat files.
 // Any failure prints the reason and returns 1, the status of
 // process_exit(1); the lattice is released first.
 #include <math.h>
@@ -57,9 +63,14 @@ struct Run {
   std::string in_dir = "./in", out_dir = "./out", params_path;
   lbm_deck d{};
   std::vector<uint8_t> mask;
+  bool walsh = false;         // a WALSH_NS_ON + COMPUTE_ON_SOLIDS run
   std::vector<Macro> mv;      // what the host holds for output
   std::vector<Macro> init_mv; // init_problem's values (kept at solid nodes)
   long N = 0;
+
+  // a node the kernels never write: an image solid, unless every node is
+  // computed
+  bool kept(long n) const { return !walsh && mask[n] == 1; }
 
   std::string name(const char *prefix, int frame, int subs, const char *suffix) const {
     char buf[1024];
@@ -175,6 +186,8 @@ int deck_main(int argc, char **argv) {
       R.in_dir = need("--in");
     else if (s == "--out")
       R.out_dir = need("--out");
+    else if (s == "--walsh")
+      R.walsh = true;
     else if (s == "--storage") {
       std::string v = need("--storage");
       storage = v == "f64" ? LBM_STORE_F64 : v == "f32" ? LBM_STORE_F32 : v == "f16" ? LBM_STORE_F16 : -1;
@@ -185,7 +198,7 @@ int deck_main(int argc, char **argv) {
       if (compute < 0) die_msg("--compute takes f64 or f32");
     } else if (s == "-h" || s == "--help") {
       printf("usage: lbgpu_prime [name] [--dims 2|3] [--subs 1|2] [--storage f64|f32|f16] "
-             "[--compute f64|f32] [--in DIR] [--out DIR]\n");
+             "[--compute f64|f32] [--in DIR] [--out DIR] [--walsh]\n");
       return 0;
     } else if (!s.empty() && s[0] != '-' && deck_name.empty())
       deck_name = s;
@@ -194,6 +207,8 @@ int deck_main(int argc, char **argv) {
   }
   if (R.dims != 2 && R.dims != 3) die_msg("--dims must be 2 or 3");
   if (R.nsubs != 1 && R.nsubs != 2) die_msg("--subs must be 1 or 2");
+  if (R.walsh && R.dims != 2)
+    die_msg("--walsh: the reference sets ns from the 2-D solids image only (src/latman.c:413-416)");
 
   // construct_lattice (src/latman.c:91-493)
   R.params_path = deck_name.empty() ? R.in_dir + "/params.in" : "./in_" + deck_name + "/params.in";
@@ -262,7 +277,16 @@ int deck_main(int argc, char **argv) {
   HandleGuard guard;
   check(lbm_create(&p, &guard.h), "construct_lattice");
   lbm_handle *h = guard.h;
-  check(lbm_set_solids(h, R.mask.data()), "solids H2D");
+  if (R.walsh) {
+    // every node is computed; image solids take ns, the others 0
+    std::vector<uint8_t> none(R.N, 0);
+    std::vector<double> ns(R.N);
+    for (long n = 0; n < R.N; n++) ns[n] = R.mask[n] == 1 ? d.ns : 0.0;
+    check(lbm_set_solids(h, none.data()), "solids H2D");
+    check(lbm_set_ns(h, ns.data()), "ns H2D");
+  } else {
+    check(lbm_set_solids(h, R.mask.data()), "solids H2D");
+  }
 
   // init_problem, frame 0 from the host values (src/lbgpu_prime.cu:97-101)
   R.init_problem();
@@ -319,7 +343,7 @@ int deck_main(int argc, char **argv) {
     while (pest && opos < ot.size() && ot[opos] == time) {
       const long n = (long)oy[opos] * d.LX + ox[opos];
       double c = 0.0, ux = 0.0, uy = 0.0;
-      if (R.mask[n] == 1) {
+      if (R.kept(n)) {
         if (R.nsubs == 2) c = R.init_mv[1].rho[n];
       } else {
         if (R.nsubs == 2) {
@@ -345,7 +369,7 @@ int deck_main(int argc, char **argv) {
       // write them)
       const Macro &i0 = R.init_mv[s];
       for (long n = 0; n < R.N; n++)
-        if (R.mask[n] == 1) {
+        if (R.kept(n)) {
           m.rho[n] = i0.rho[n];
           m.ux[n] = i0.ux[n];
           m.uy[n] = i0.uy[n];

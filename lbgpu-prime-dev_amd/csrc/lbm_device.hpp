@@ -118,6 +118,7 @@ struct StepArgs {
   CT rhoA_s, inv_tau_s, sink_fac;
   CT scale[2], inv_scale[2], c0;
   ZfaceFuse zb;   // zb.on = 0: no boundary work in the half-step
+  const double *ns;  // [nf] Walsh partial bounce-back fraction (WALSH kernels)
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs; map them so that
@@ -227,8 +228,12 @@ __device__ __forceinline__ void scatter2(const StepArgs<ST, CT> &A, int idx, con
 // order of src/k_stream_collide_stream.c:80-390 and
 // src/lattice.c:2850-2944.  SUBS 0: He-Luo incompressible flow on shifted
 // populations; SUBS 1: Inamuro solute advected by the flow's u_eq.
-template <int Q, typename ST, typename CT, int SUBS, bool ODD>
-__device__ __forceinline__ void collide(const StepArgs<ST, CT> &A, int idx, CT (&f)[Q], CT (&u)[3]) {
+// WALSH: partial bounce-back (src/k_stream_collide_stream.c:129-135,183-192,
+// src/lattice.c:2936-2939): the momentum and the force enter u scaled by
+// kns = 1 - ns; the caller mixes the opposite pre-collision populations in.
+template <int Q, typename ST, typename CT, int SUBS, bool ODD, bool WALSH = false>
+__device__ __forceinline__ void collide(const StepArgs<ST, CT> &A, int idx, CT (&f)[Q], CT (&u)[3],
+                                        CT kns = (CT)1) {
   constexpr int D = Q == 9 ? 2 : 3;
   CT rho = f[0];
 #pragma unroll
@@ -243,10 +248,19 @@ __device__ __forceinline__ void collide(const StepArgs<ST, CT> &A, int idx, CT (
       u[1] += ((CT)CY(a)) * f[a];
       if (D == 3) u[2] += ((CT)CZ(a)) * f[a];
     }
+    if constexpr (WALSH) {
+#pragma unroll
+      for (int d = 0; d < D; d++) u[d] *= kns;
+    }
 #pragma unroll
     for (int d = 0; d < D; d++) u[d] /= A.rhoA;
+    if constexpr (WALSH) {
 #pragma unroll
-    for (int d = 0; d < D; d++) u[d] += A.tau * A.g[d];
+      for (int d = 0; d < D; d++) u[d] += kns * A.tau * A.g[d];
+    } else {
+#pragma unroll
+      for (int d = 0; d < D; d++) u[d] += A.tau * A.g[d];
+    }
   } else {
     if (!ODD) rho *= A.sink_fac;
     rho *= A.sink_fac;
@@ -392,7 +406,20 @@ __device__ __forceinline__ void zface_fix(const StepArgs<ST, CT> &A, int idx, CT
   }
 }
 
-template <int Q, int NS, typename ST, typename CT, bool ODD, bool OFF32>
+// Walsh partial bounce-back: post-collision (1 - ns) BGK + ns times the
+// opposite pre-collision population (src/k_stream_collide_stream.c:363-383,
+// src/k_collide.c:373-392; Walsh, Burwinkle & Saar 2009)
+template <int Q, typename CT>
+__device__ __forceinline__ void walsh_mix(CT (&f)[Q], const CT (&fp)[Q], CT ns, CT kns) {
+  f[0] = f[0] * kns + ns * fp[0];
+#pragma unroll
+  for (int a = 1; a < Q; a += 2) {
+    f[a] = f[a] * kns + ns * fp[a + 1];
+    f[a + 1] = f[a + 1] * kns + ns * fp[a];
+  }
+}
+
+template <int Q, int NS, typename ST, typename CT, bool ODD, bool OFF32, bool WALSH = false>
 __global__ __launch_bounds__(kBlock) void k_half_step(StepArgs<ST, CT> A) {
   const int idx = xcd_block(A.nblk8) * kBlock + (int)threadIdx.x;
   if (idx >= A.nf) return;
@@ -400,13 +427,32 @@ __global__ __launch_bounds__(kBlock) void k_half_step(StepArgs<ST, CT> A) {
   nb[0] = idx;
   if (ODD) neighbors<Q>(A.G, idx, nb);
   CT f[Q], u[3];
+  CT ns = (CT)0, kns = (CT)1;
+  if constexpr (WALSH) {
+    const double nsd = A.ns[idx];
+    ns = (CT)nsd;
+    kns = (CT)(1. - nsd);
+  }
   if (NS == 2 && A.solute_on) {
     CT g[Q];
     gather2<Q, ST, CT, ODD, OFF32>(A, idx, nb, f, g);
     if constexpr (Q == 19)
       if (zface_node<Q>(A, idx)) zface_fix<ST, CT, true>(A, idx, f, g);
-    collide<Q, ST, CT, 0, ODD>(A, idx, f, u);
-    collide<Q, ST, CT, 1, ODD>(A, idx, g, u);
+    if constexpr (WALSH) {
+      CT fp[Q], gp[Q];
+#pragma unroll
+      for (int a = 0; a < Q; a++) {
+        fp[a] = f[a];
+        gp[a] = g[a];
+      }
+      collide<Q, ST, CT, 0, ODD, true>(A, idx, f, u, kns);
+      collide<Q, ST, CT, 1, ODD, true>(A, idx, g, u, kns);
+      walsh_mix<Q, CT>(f, fp, ns, kns);
+      walsh_mix<Q, CT>(g, gp, ns, kns);
+    } else {
+      collide<Q, ST, CT, 0, ODD>(A, idx, f, u);
+      collide<Q, ST, CT, 1, ODD>(A, idx, g, u);
+    }
     scatter2<Q, ST, CT, ODD, OFF32>(A, idx, nb, f, g);
     return;
   }
@@ -414,7 +460,15 @@ __global__ __launch_bounds__(kBlock) void k_half_step(StepArgs<ST, CT> A) {
   gather<Q, NS, ST, CT, 0, ODD, OFF32>(A, idx, nb, f);
   if constexpr (Q == 19)
     if (zface_node<Q>(A, idx)) zface_fix<ST, CT, false>(A, idx, f, f);
-  collide<Q, ST, CT, 0, ODD>(A, idx, f, u);
+  if constexpr (WALSH) {
+    CT fp[Q];
+#pragma unroll
+    for (int a = 0; a < Q; a++) fp[a] = f[a];
+    collide<Q, ST, CT, 0, ODD, true>(A, idx, f, u, kns);
+    walsh_mix<Q, CT>(f, fp, ns, kns);
+  } else {
+    collide<Q, ST, CT, 0, ODD>(A, idx, f, u);
+  }
   scatter<Q, NS, ST, CT, 0, ODD, OFF32>(A, idx, nb, f);
 }
 

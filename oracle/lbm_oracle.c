@@ -31,6 +31,11 @@
  *       - halo mode with periodic self-exchange equals the wrapped run bitwise.
  *   reductions (compute.c)
  *       - brute-force numpy over the same mv arrays.
+ *   Walsh partial bounce-back (WALSH_NS_ON path)
+ *       - ns = 0 reproduces the plain update bitwise; ns = 1 reverses every
+ *         population, so the state repeats exactly every two steps; mass is
+ *         conserved for any ns; in a uniform-ns periodic box the total
+ *         momentum follows J' = J ((1-ns)(1-w ns) - ns) + (1-ns)^2 rho_A g N.
  */
 #include "lbm_oracle.h"
 
@@ -63,6 +68,7 @@ struct orc {
   double inv_tau[2];
   int time;
   double flux, fmass;
+  double *ns;           /* [N] Walsh partial bounce-back fraction, or NULL */
 };
 
 static int g_threads = 0;
@@ -140,7 +146,18 @@ void orc_destroy(orc *o) {
   free(o->f);
   free(o->mv);
   free(o->sol);
+  free(o->ns);
   free(o);
+}
+
+int orc_set_ns(orc *o, const double *ns) {
+  free(o->ns);
+  o->ns = NULL;
+  if (!ns) return 0;
+  o->ns = (double *)malloc(sizeof(double) * o->N);
+  if (!o->ns) return 1;
+  memcpy(o->ns, ns, sizeof(double) * o->N);
+  return 0;
 }
 
 void orc_set_time(orc *o, int t) { o->time = t; }
@@ -291,11 +308,19 @@ static void update_node(orc *o, int odd, int t, int write_mv, int i, int j, int 
         u[1] += ((double)VY[a]) * f[a];
         if (D == 3) u[2] += ((double)VZ[a]) * f[a];
       }
+      /* Walsh partial bounce-back scales the momentum by 1 - ns
+       * (src/k_stream_collide_stream.c:129-135,183-192) */
+      if (o->ns)
+        for (a = 0; a < D; a++) u[a] *= (1. - o->ns[n]);
       for (a = 0; a < D; a++) u[a] /= p->rho_A[s];
     }
-    /* body force / sink: src/k_stream_collide_stream.c:237-258 */
+    /* body force / sink: src/k_stream_collide_stream.c:237-258; with ns
+     * the force too is scaled by 1 - ns (apply_accel_mv, src/lattice.c:2936-2939) */
     if (s != 1) {
-      for (a = 0; a < D; a++) u[a] += p->tau[s] * p->gaccel[s * 3 + a];
+      if (o->ns)
+        for (a = 0; a < D; a++) u[a] += (1. - o->ns[n]) * p->tau[s] * p->gaccel[s * 3 + a];
+      else
+        for (a = 0; a < D; a++) u[a] += p->tau[s] * p->gaccel[s * 3 + a];
     } else {
       rho *= (1.0 - p->sink * p->tau[1]);
     }
@@ -315,6 +340,9 @@ static void update_node(orc *o, int odd, int t, int write_mv, int i, int j, int 
       usq = u[0] * u[0] + u[1] * u[1];
       if (D == 3) usq += u[2] * u[2];
     }
+    double fpre[19];
+    if (o->ns)
+      for (a = 0; a < Q; a++) fpre[a] = f[a];
     for (a = 0; a < Q; a++) {
       double vdotu;
       f[a] *= (1. - o->inv_tau[s]);
@@ -325,6 +353,19 @@ static void update_node(orc *o, int odd, int t, int write_mv, int i, int j, int 
       } else {
         f[a] += o->wt[a] * rho * o->inv_tau[s];
         f[a] += o->wt[a] * p->rho_A[s] * (3. * vdotu + 4.5 * vdotu * vdotu - 1.5 * usq) * o->inv_tau[s];
+      }
+    }
+    /* Walsh: post-collision (1 - ns) BGK + ns times the opposite pre-collision
+     * population (src/k_stream_collide_stream.c:363-383, src/k_collide.c:373-392) */
+    if (o->ns) {
+      const double ns = o->ns[n];
+      f[0] *= 1. - ns;
+      f[0] += ns * fpre[0];
+      for (a = 1; a < Q; a += 2) {
+        f[a] *= 1. - ns;
+        f[a] += ns * fpre[a + 1];
+        f[a + 1] *= 1. - ns;
+        f[a + 1] += ns * fpre[a];
       }
     }
     if (odd) {

@@ -74,6 +74,9 @@ typedef struct orc orc;
 
 orc *orc_create(const orc_params *p);
 void orc_destroy(orc *o);
+/* Walsh partial bounce-back (Walsh, Burwinkle & Saar 2009; the reference's
+ * WALSH_NS_ON path): per node a fraction ns in [0, 1]; NULL turns it off */
+int orc_set_ns(orc *o, const double *ns);
 int orc_num_threads(void);
 void orc_set_threads(int n);
 

@@ -84,6 +84,8 @@ def lib():
                                             ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_int), dp, dp,
                                             dp, dp, dp, dp]
+        L.orc_set_ns.restype = ctypes.c_int
+        L.orc_set_ns.argtypes = [vp, dp]
         L.orc_num_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -212,6 +214,16 @@ class Oracle:
     @time.setter
     def time(self, t):
         lib().orc_set_time(self.h, t)
+
+    def set_ns(self, ns):
+        """Walsh partial bounce-back fraction per node (None: off)"""
+        if ns is None:
+            lib().orc_set_ns(self.h, None)
+            return
+        self._ns = np.ascontiguousarray(ns, dtype=np.float64).reshape(-1)
+        assert self._ns.size == self.N
+        if lib().orc_set_ns(self.h, _dp(self._ns)):
+            raise MemoryError("orc_set_ns")
 
     def step(self, write_mv=False):
         lib().orc_step(self.h, int(write_mv))

@@ -320,6 +320,8 @@ def test_deck_main_argument_errors(io, tmp_path):
     assert io.deck_main("--dims", "4") == 1
     assert io.deck_main("--in", str(tmp_path / "missing")) == 1
     assert io.deck_main("--help") == 0
+    # the reference's Walsh build sets ns from the 2-D image only
+    assert io.deck_main("--dims", "3", "--walsh", "--in", str(tmp_path)) == 1
 
 
 def test_observations_read_filter_and_order(io, tmp_path):

@@ -166,3 +166,54 @@ def test_deck_run_matches_oracle(tmp_path, nsubs):
     if nsubs == 2:
         # the solute left the cube and reached the fluid around it
         assert o.mv(1)[0][fluid & ~cube.reshape(-1)].max() > 0
+
+
+def test_walsh_deck_matches_oracle(tmp_path):
+    """--walsh: the image's black pixels carry params.in's ns and every node
+    is computed (a WALSH_NS_ON + COMPUTE_ON_SOLIDS build); the .dat files
+    keep the image's solids as zeros, the text files hold every node"""
+    say("case deck main --walsh")
+    io = _pkg.load_submodule("deck")
+    LX, LY, FR, NF = 32, 24, 5, 2
+    rng = np.random.default_rng(71)
+    mask = (rng.random((LY, LX)) < 0.25).astype(np.uint8)
+    mask[:2] = 0
+    mask[-2:] = 0
+    ind = tmp_path / "in"
+    ind.mkdir()
+    write_bmp24(str(ind / f"{LX}x{LY}.bmp"), mask)
+    edits = {"LX": LX, "LY": LY, "BX": 16, "BY": 8, "NumFrames": NF, "FrameRate": FR, "tau[0]": "0.8",
+             "gforce_x[0]": "0.00001", "tau[1]": "0.7", "ns": "0.35", "sigma_t_on": 0, "sigma_t_off": -1,
+             "x1": 10, "x2": 20, "y1": 8, "y2": 16, "rho_sigma": "1.0", "pressure_n_in[0]": 0,
+             "pressure_s_in[0]": 0}
+    write_deck(str(ind / "params.in"), edits)
+    out = tmp_path / "out"
+    out.mkdir()
+    assert io.deck_main("--dims", "2", "--subs", "2", "--in", str(ind), "--out", str(out), "--walsh") == 0
+    assert io.deck_main("--dims", "3", "--in", str(ind), "--out", str(out), "--walsh") == 1
+
+    cube = np.zeros((LY, LX), bool)
+    cube[8:17, 10:21] = True
+    deck = io.read_deck(str(ind / "params.in"), 2, 2)
+    kw = dict(dims=2, ni=LX, nj=LY, nsubs=2, tau=(0.8, 0.7), rho_A=(1.0, 1.0), gaccel=(1e-5, 0, 0, 0, 0, 0),
+              sink=deck.sink, sigma_t_on=0, sigma_t_off=NF * FR,
+              zeroconcgrad_n=deck.zeroconcgrad_n, zeroconcgrad_s=deck.zeroconcgrad_s)
+    o = Oracle(**kw)
+    o.set_solids(np.zeros_like(mask))
+    o.set_ns(np.where(mask == 1, 0.35, 0.0))
+    o.init_rest(np.where(cube, 1.0, 0.0).reshape(-1))
+    solid = mask.reshape(-1) == 1
+    for frame in range(1, NF + 1):
+        o.run(FR)
+        for s in range(2):
+            mo = o.mv(s)
+            for key, want in zip(("rho", "u_x", "u_y"), mo):
+                got = np.fromfile(io.filename(str(out), 2, LX, LY, 2, key, frame, s, ".dat"), np.float64)
+                assert np.array_equal(got[~solid], want[~solid]), (frame, s, key)
+                assert not got[solid].any()
+            for key, want in zip(("rho", "ux", "uy"), mo):
+                got = read_txt(io.filename(str(out), 2, LX, LY, 2, key, frame, s, ".txt"), LX, LY)
+                assert np.max(np.abs(got - want)) <= 1e-15, (frame, s, key)
+        say(f"walsh frame {frame} matches")
+    # the partially solid pixels moved: they are computed, not held
+    assert np.abs(o.mv(0)[1][solid]).max() > 0
