@@ -262,6 +262,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variants-only", action="store_true",
                     help="of the extras, run only the 256^3 flow + solute storage and Walsh variants")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="of the extras, run only the 128^3 discrete-adjoint workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
@@ -288,7 +290,7 @@ def main():
     dom_ms = max(odd_ms, even_ms)
     achieved = B / (dom_ms * 1e-3)
     step_bw = B * args.steps / wall
-    traffic, traffic_src = recorded_traffic(f"k_half_step<19, 1, double, double, {'true' if dom == 'odd' else 'false'}, true>")
+    traffic, traffic_src = recorded_traffic(f"k_half_step<19, 1, double, double, {'true' if dom == 'odd' else 'false'}, true, false>")
     result = {
         "metric": METRIC,
         "value": round(mlups, 2),
@@ -326,7 +328,9 @@ def main():
         },
     }
 
-    if not args.no_extra:
+    if args.adjoint_only:
+        result["extra"]["adjoint_c4"] = adjoint_timing(L)
+    elif not args.no_extra:
         variants = [("flow_solute_fp64", 0, 0, 8), ("flow_solute_fp32_shifted", 1, 1, 4),
                     ("flow_solute_fp32_shifted_f64_collision", 1, 0, 4), ("flow_solute_fp16_scaled", 2, 1, 2)]
         ext = {}

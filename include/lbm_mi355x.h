@@ -178,7 +178,7 @@ int lbm_reduce(lbm_handle *h, int kind, int subs, double *out);
 int lbm_last_flux(lbm_handle *h, double *flux, double *mass);
 
 /* geometry numbers: N, fluid nodes, population slots per direction
- * (fluid nodes and halo ghost entries rounded up to 64-node tiles) */
+ * (fluid nodes and halo ghost entries rounded up to 2048-node tiles) */
 int lbm_info(const lbm_handle *h, long *n_nodes, long *n_fluid, long *row_stride);
 
 /* Kernel timing with HIP events on the library's stream.  While enabled

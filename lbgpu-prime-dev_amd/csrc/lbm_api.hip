@@ -57,7 +57,7 @@ struct lbm_handle {
   int D, Q, U, L;  // dims, directions, unbound directions, last-axis length
   long N, E;
   int nf = 0;
-  long tiles = 0;  // 64-node tiles incl. ghost entries
+  long tiles = 0;  // kTile-node tiles incl. ghost entries
   int W = 0;       // 64-node words per lattice row
   int nblk8 = 0;
   size_t st_bytes = 8;

@@ -1,6 +1,6 @@
 // Device code of the stream-collide library (gfx950, wave64).
 //
-// Storage: populations of fluid nodes only, in tiles of 64 consecutive
+// Storage: populations of fluid nodes only, in tiles of kTile consecutive
 // compact nodes (lbm_index.hpp), compact order = dense order (i fastest);
 // with a solute, the two populations of a slot are adjacent.
 // Slot meaning follows the reference's in-place AA scheme (Bailey et al.,
@@ -105,7 +105,7 @@ struct ZfaceFuse {
 
 template <typename ST, typename CT>
 struct StepArgs {
-  ST *F;          // tiles [nf/64][NS][Q][64]: populations the half-step reads
+  ST *F;          // tiles [nf/kTile][Q][kTile][NS]: populations the half-step reads
   ST *Fo;         // where it writes them (== F in place; another state
                   // buffer when the adjoint replays a segment)
   Geo G;
