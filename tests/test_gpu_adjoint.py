@@ -51,12 +51,13 @@ def test_adjoint_matches_tangent_linear_model():
         dp[q] = 1.0
         _, dJ = tlm_objective(mask, 1.001, 1.0, zf, 60, 20, PARAMS, cmap, dp, np.zeros_like(cmap))
         say(f"param {q}: adjoint {grad[q]:.12e} tlm {dJ:.12e}")
-        assert abs(grad[q] - dJ) <= 1e-9 * max(abs(dJ), 1e-30) + 1e-14
+        say(f"param {q}: relative difference {abs(grad[q] - dJ) / max(abs(dJ), 1e-300):.2e}")
+        assert abs(grad[q] - dJ) <= 1e-12 * abs(dJ) + 1e-22
     dc = np.random.default_rng(9).uniform(-1, 1, cmap.shape)
     _, dJ = tlm_objective(mask, 1.001, 1.0, zf, 60, 20, PARAMS, cmap, np.zeros(5), dc)
     adj = float(np.sum(gmap * dc))
     say(f"inlet map direction: adjoint {adj:.12e} tlm {dJ:.12e}")
-    assert abs(adj - dJ) <= 1e-9 * abs(dJ)
+    assert abs(adj - dJ) <= 1e-12 * abs(dJ)
     # the map gradient lives on the inlet face's fluid nodes only
     assert np.all(gmap[mask[0] == 1] == 0.0)
 
@@ -131,12 +132,13 @@ def test_observation_adjoint_matches_tangent_linear_model(window):
         _, dJ = tlm_objective(mask, 1.001, 1.0, zf, 60, window, PARAMS, cmap, dp, np.zeros_like(cmap),
                               observations=obs)
         say(f"param {q}: adjoint {grad[q]:.12e} tlm {dJ:.12e}")
-        assert abs(grad[q] - dJ) <= 1e-9 * max(abs(dJ), 1e-30) + 1e-14
+        say(f"param {q}: relative difference {abs(grad[q] - dJ) / max(abs(dJ), 1e-300):.2e}")
+        assert abs(grad[q] - dJ) <= 1e-12 * abs(dJ) + 1e-22
     dc = np.random.default_rng(19).uniform(-1, 1, cmap.shape)
     _, dJ = tlm_objective(mask, 1.001, 1.0, zf, 60, window, PARAMS, cmap, np.zeros(5), dc, observations=obs)
     adj = float(np.sum(gmap * dc))
     say(f"inlet map direction: adjoint {adj:.12e} tlm {dJ:.12e}")
-    assert abs(adj - dJ) <= 1e-9 * abs(dJ)
+    assert abs(adj - dJ) <= 1e-12 * abs(dJ)
     # a second sweep of the same run gives the same gradient (the seeds are
     # cleared after their step)
     grad2, gmap2 = lat.adjoint()

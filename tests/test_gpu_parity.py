@@ -131,6 +131,35 @@ def test_2d_reference_bcs_with_solute(phase_kw):
     compare(o, g, mask, 2, "2D Zou-He + zero gradient")
 
 
+@pytest.mark.parametrize("nsubs", [1, 2])
+def test_2d_reference_bcs_at_walled_corners(nsubs):
+    """side walls reaching the boundary rows and solids next to them: the
+    odd-phase unknowns whose wrapped source is solid go to the boundary
+    node's own slot, which the interior node beside it pulls (the
+    reference's set_f1d_d fallback)"""
+    say(f"case 2D BC corners nsubs={nsubs}")
+    rng = np.random.default_rng(88 + nsubs)
+    ni, nj = 11, 12
+    mask = np.zeros((nj, ni), np.uint8)
+    mask[:, 0] = 1
+    mask[:, -1] = 1
+    mask[1, 4] = 1
+    mask[nj - 2, 7] = 1
+    mask[1, 2] = 1
+    mask[nj - 2, ni - 3] = 1
+    mask[5, 5] = 1
+    kw = dict(dims=2, ni=ni, nj=nj, nsubs=nsubs, tau=(0.8, 0.75), pressure_s_in=(1, 0), pressure_n_out=(1, 0),
+              rho_in=1.002, rho_out=0.999, zeroconcgrad_n=nsubs == 2, zeroconcgrad_s=nsubs == 2)
+    o, g = build_pair(mask, **kw)
+    f, gg = random_state(rng, 9, mask.size, nsubs)
+    load_both(o, g, f, gg)
+    for t in range(31):
+        o.step(True)
+        g.step(True)
+    _, bitwise = compare(o, g, mask, nsubs, f"2D BC corners nsubs={nsubs}")
+    assert bitwise
+
+
 @pytest.mark.parametrize("halo", [0, 1])
 def test_3d_zface_pressure_solute_flux(halo):
     say(f"case 3D z-face BCs halo={halo}")
