@@ -6,9 +6,8 @@ solute from the Dirichlet inlet against the erfc closed form.
 Zou-He pressure faces with halfway bounce-back walls leave a period-2 mode
 in the plane sums of u_z (alternating between neighbouring planes and from
 step to step; about +-4 % in this duct, +-1.3 % for textbook Zou-He in a
-2-D channel); it is exactly conserved by streaming and collision in the
-bulk, so it does not decay.  The checks therefore use the mean of two
-neighbouring planes, in which the mode cancels."""
+2-D channel) that persists unchanged at steady state.  The checks therefore
+use the mean of two neighbouring planes, in which the mode cancels."""
 from math import erfc, sqrt
 
 import numpy as np
