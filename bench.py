@@ -254,7 +254,7 @@ def cpu_baseline(mask, kw, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -325,6 +325,10 @@ def main():
             "copy_gbps": round(copy_gbps, 1),
             "dominant_frac_of_copy": round(achieved / (copy_gbps * 1e9), 4),
             "mass_sum": sums[0],
+            # Darcy permeability after warmup + steps (the flow workload is
+            # specified as 5,000 steps), L = LZ - 1 between the face planes
+            "permeability": permeability(sums, N, 0.8, 1.001, 1.0, n - 1),
+            "permeability_after_steps": args.warmup + args.steps,
         },
     }
 
