@@ -194,11 +194,21 @@ int lbm_profile_read(lbm_handle *h, double *ms, int *count);
 
 /* Objective of the discrete adjoint (no reference counterpart): runs
  * nsteps steps from the current state like lbm_run, keeping a copy of the
- * state every `every` steps, and returns
- *   J = 1/(window * LX * LY) * sum over the last `window` steps of
- *       sum over the fluid nodes of plane flux_plane of sum_a c_a,z g_a
- * (the solute flux through that plane, pre-collision, after the BCs).
+ * state every `every` steps, and returns J, the window term selected by
+ * lbm_objective_select over the last `window` steps:
+ *   LBM_OBJ_OUTLET_FLUX (default): 1/(window * LX * LY) * sum over the
+ *       steps of sum over the fluid nodes of plane flux_plane of
+ *       sum_a c_a,z g_a (the solute flux through that plane, pre-collision,
+ *       after the BCs);
+ *   LBM_OBJ_PERMEABILITY: the Darcy permeability rho_A nu <u_z,phys> / G
+ *       averaged over the steps, nu = (tau - 1/2)/3, u_phys = u_eq -
+ *       (tau - 1/2) g the flow velocity averaged over all N nodes (solids
+ *       count as 0), G = (rho_bot - rho_top)/(3 (LZ - 1)) with the z-face
+ *       pressure conditions, plus rho_A g_z (the driving gradient; it must
+ *       not be 0).
  * D3Q19 flow + solute, FP64, halo_mode 0, sink 0. */
+enum lbm_objective_kind { LBM_OBJ_OUTLET_FLUX = 0, LBM_OBJ_PERMEABILITY = 1 };
+int lbm_objective_select(lbm_handle *h, int kind);
 int lbm_objective_run(lbm_handle *h, int nsteps, int window, int every, double *J);
 /* Observation points of the objective (no reference counterpart; the
  * reference's PEST output samples the same quantity, src/latman.c:1072-1190):

@@ -19,6 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "liblbm_mi355x.so")
 
 STORE_F64, STORE_F32, STORE_F16 = 0, 1, 2
+OBJECTIVES = {"flux": 0, "permeability": 1}
 COMPUTE_F64, COMPUTE_F32 = 0, 1
 REDUCE_REF, REDUCE_SUMS = 0, 1
 MACRO_NONE, MACRO_FRAME, MACRO_PEST = 0, 1, 2
@@ -89,6 +90,7 @@ def lib():
             "lbm_download_macro": [vp, ip, dp, dp, dp, dp],
             "lbm_upload_macro": [vp, ip, dp, dp, dp, dp],
             "lbm_set_ns": [vp, dp],
+            "lbm_objective_select": [vp, ip],
             "lbm_sample_macro": [vp, ip, ip, ctypes.POINTER(ctypes.c_long), dp],
             "lbm_reduce": [vp, ip, ip, dp],
             "lbm_last_flux": [vp, dp, dp],
@@ -338,9 +340,12 @@ class Lattice:
             self.h, node.size, node.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
             step.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(target), _dp(w) if w is not None else None))
 
-    def objective(self, nsteps, window, every=100):
-        """J over the last `window` of `nsteps` steps, plus the observation
-        misfit when observations are set (see lbm_objective_run)"""
+    def objective(self, nsteps, window, every=100, kind="flux"):
+        """J over the last `window` of `nsteps` steps (kind "flux": the solute
+        flux through flux_plane; "permeability": rho_A nu <u_z,phys> / G),
+        plus the observation misfit when observations are set (see
+        lbm_objective_run)"""
+        _check(lib().lbm_objective_select(self.h, OBJECTIVES[kind]))
         J = ctypes.c_double()
         _check(lib().lbm_objective_run(self.h, int(nsteps), int(window), int(every), ctypes.byref(J)))
         return J.value

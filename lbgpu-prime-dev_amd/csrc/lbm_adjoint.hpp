@@ -11,10 +11,12 @@
 // nodes with linear combinations of the known ones; its transpose adds the
 // unknowns' adjoint to the known ones and clears the unknowns.
 //
-// Objective: J = 1/(W A) sum over the last W steps of the solute flux
+// Objectives: J = 1/(W A) sum over the last W steps of the solute flux
 // sum_a c_a,z g_a through plane zf (pre-collision, after the BCs), A =
-// LX*LY.  Parameters: body force g, tau, tau_s and the inlet concentration
-// map c_b(x, y).
+// LX*LY; or the permeability rho_A nu <u_z,phys> / G over the last W steps,
+// which is linear in the flow's z momentum sum_a c_a,z f_a of every node
+// (plus terms in tau and g_z the host adds).  Parameters: body force g,
+// tau, tau_s and the inlet concentration map c_b(x, y).
 #pragma once
 
 #include "lbm_aux.hpp"
@@ -27,6 +29,8 @@ struct AdjArgs {
                                // values are formed here from their known ones
   double *L;                   // adjoint state, same tile layout as the populations
   double seed;                 // dJ / d(plane flux of this step); 0 outside the window
+  double kseed;                // dJ / d(z momentum of a node, this step); 0 outside the
+                               // window or for the flux objective
   int zf;                      // flux plane
   double *gacc;                // [launched blocks][5]: dJ/dg_x, g_y, g_z, tau, tau_s
   double *cgrad;               // [LY][LX] dJ/dc_bot_map (face-node kernel)
@@ -196,7 +200,7 @@ __device__ __forceinline__ void adj_new(const AdjArgs &J, const AdjOut &o, doubl
   lb = (1.0 - om) * lam + om * o.Lr;
   if (CX(B_) != 0) lb += CX(B_) > 0 ? o.Gr[0] : -o.Gr[0];
   if (CY(B_) != 0) lb += CY(B_) > 0 ? o.Gr[1] : -o.Gr[1];
-  if (CZ(B_) != 0) lb += CZ(B_) > 0 ? o.Gr[2] : -o.Gr[2];
+  if (CZ(B_) != 0) lb += CZ(B_) > 0 ? o.Gr[2] + J.kseed : -o.Gr[2] - J.kseed;
   mb = (1.0 - oms) * mu + oms * o.Sg;
   if (CZ(B_) != 0) mb += CZ(B_) > 0 ? seed : -seed;
   mb += oseed;

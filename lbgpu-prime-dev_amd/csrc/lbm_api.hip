@@ -91,10 +91,12 @@ struct lbm_handle {
   // run of the same lattice reuses them) and are released with the lattice.
   struct {
     int valid = 0;
+    int kind = LBM_OBJ_OUTLET_FLUX;  // the window term (lbm_objective_select)
     int t0 = 0, nsteps = 0, window = 0, every = 0;
     std::vector<void *> ckpt;  // state before steps t0+1, t0+every+1, ...
     double *jacc = nullptr;
     double J = 0.0;
+    double Jwin = 0.0;         // the window term of J
     std::vector<void *> seg;   // adjoint: the states of one replayed segment
     double *L = nullptr;       // adjoint state
     double *gacc = nullptr;    // per-block parameter gradients
@@ -1008,7 +1010,8 @@ static int adjoint_supported(const lbm_handle *h) {
   if (p.storage != LBM_STORE_F64 || p.compute != LBM_COMPUTE_F64) return lbm_fail("the adjoint runs in FP64");
   if (p.halo_mode) return lbm_fail("the adjoint needs the periodic (halo_mode = 0) lattice");
   if (p.sink != 0.0) return lbm_fail("the adjoint needs sink = 0");
-  if (p.flux_plane < 0 || p.flux_plane >= p.nk) return lbm_fail("the objective needs flux_plane");
+  if (h->obj.kind == LBM_OBJ_OUTLET_FLUX && (p.flux_plane < 0 || p.flux_plane >= p.nk))
+    return lbm_fail("the objective needs flux_plane");
   if (h->ns) return lbm_fail("the adjoint does not cover partial bounce-back (lbm_set_ns)");
   return 0;
 }
@@ -1053,6 +1056,24 @@ extern "C" int lbm_objective_observations(lbm_handle *h, int n, const long *node
   return 0;
 }
 
+// permeability objective: rho_A nu <u_z,phys> / G over the window, with
+// nu = (tau - 1/2)/3, u_phys = u_eq - (tau - 1/2) g (superficial average over
+// all N nodes) and G = (rho_bot - rho_top)/(3 (LZ - 1)) (z-face pressure
+// conditions) + rho_A g_z, the driving gradient
+static double perm_drive(const lbm_handle *h) {
+  const lbm_params &p = h->p;
+  const double dp = p.zface_pressure ? (p.rho_bot - p.rho_top) / (3.0 * (p.nk - 1)) : 0.0;
+  return dp + p.rho_A[0] * p.gaccel[2];
+}
+
+extern "C" int lbm_objective_select(lbm_handle *h, int kind) {
+  if (!h) return lbm_fail("null handle");
+  if (kind != LBM_OBJ_OUTLET_FLUX && kind != LBM_OBJ_PERMEABILITY) return lbm_fail("unknown objective %d", kind);
+  h->obj.kind = kind;
+  h->obj.valid = 0;
+  return 0;
+}
+
 extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int every, double *J) {
   if (need_lattice(h) || adjoint_supported(h)) return 1;
   const int nobs = (int)h->obj.o_step.size();
@@ -1063,6 +1084,9 @@ extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int ever
   const int t0 = h->time;
   if (t0 + 1 < h->p.sigma_t_on || t0 + nsteps > h->p.sigma_t_off)
     return lbm_fail("the solute must be active for the whole objective run");
+  const bool perm = h->obj.kind == LBM_OBJ_PERMEABILITY;
+  if (perm && window > 0 && !(perm_drive(h) != 0.0))
+    return lbm_fail("the permeability needs a driving gradient (z-face pressure difference or g_z)");
   h->obj.valid = 0;
   const size_t sb = state_bytes(h);
   const size_t nck = (size_t)((nsteps + every - 1) / every);
@@ -1086,9 +1110,19 @@ extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int ever
     const bool in_window = q >= nsteps - window;
     int oe = ob;
     while (oe < nobs && o.o_step[oe] == q + 1) oe++;
-    // an observed step writes the solute concentration (the PEST switch)
-    const int macro = q == nsteps - 1 ? LBM_MACRO_FRAME : oe > ob ? LBM_MACRO_PEST : LBM_MACRO_NONE;
-    if (do_step(h, macro, in_window ? o.jacc : nullptr)) return 1;
+    // an observed step writes the solute concentration (the PEST switch);
+    // a window step of the permeability writes the flow velocity
+    const int macro = q == nsteps - 1 || (perm && in_window) ? LBM_MACRO_FRAME
+                      : oe > ob                               ? LBM_MACRO_PEST
+                                                              : LBM_MACRO_NONE;
+    if (do_step(h, macro, in_window && !perm ? o.jacc : nullptr)) return 1;
+    if (perm && in_window) {
+      // sum of u_z over the fluid nodes, added to the running sum
+      hipLaunchKernelGGL(k_sum_partial, dim3(kRedBlocks), dim3(256), 0, h->stream, h->mv + (long)3 * h->nf,
+                         (long)h->nf, h->red);
+      hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, h->stream, h->red, kRedBlocks, o.jacc);
+      HIP_OK(hipGetLastError());
+    }
     if (oe > ob) {
       hipLaunchKernelGGL(k_obs_gather, dim3(blocks_for(oe - ob)), dim3(256), 0, h->stream, mv_c,
                          o.o_node_d + ob, oe - ob, o.o_c + ob);
@@ -1140,7 +1174,17 @@ extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int ever
   o.nsteps = nsteps;
   o.window = window;
   o.every = every;
-  o.J = (window > 0 ? acc / ((double)window * (double)h->E) : 0.0) + Jobs;
+  if (window == 0)
+    o.Jwin = 0.0;
+  else if (perm) {
+    const lbm_params &p = h->p;
+    const double nu = (p.tau[0] - 0.5) / 3.0;
+    const double su = acc - (double)window * (double)h->nf * (p.tau[0] - 0.5) * p.gaccel[2];
+    o.Jwin = p.rho_A[0] * nu * su / ((double)window * (double)h->N * perm_drive(h));
+  } else {
+    o.Jwin = acc / ((double)window * (double)h->E);
+  }
+  o.J = o.Jwin + Jobs;
   o.valid = 1;
   *J = o.J;
   return 0;
@@ -1194,7 +1238,11 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
     (void)hipMemsetAsync(gacc, 0, (size_t)nslot * 5 * sizeof(double), h->stream);
     (void)hipMemsetAsync(cgrad, 0, (size_t)h->E * sizeof(double), h->stream);
   }
-  const double seed = 1.0 / ((double)W_ * (double)h->E);
+  const bool perm = h->obj.kind == LBM_OBJ_PERMEABILITY;
+  const double nu = (h->p.tau[0] - 0.5) / 3.0;
+  // dJ/d(plane flux) or dJ/d(z momentum of a node) on a window step
+  const double seed = perm ? 0.0 : 1.0 / ((double)W_ * (double)h->E);
+  const double kseed = perm && W_ > 0 ? nu / ((double)W_ * (double)h->N * perm_drive(h)) : 0.0;
   const bool off32 = !h->off64 && sb <= 0xffffffffull;
   for (int sgi = nseg - 1; sgi >= 0 && !rc; sgi--) {
     const int ts = t0 + sgi * K;                 // state seg[0] is before step ts+1
@@ -1238,6 +1286,7 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
       if (fz) set_zb(h, J.A);
       J.L = L;
       J.seed = t > t0 + T - W_ ? seed : 0.0;
+      J.kseed = t > t0 + T - W_ ? kseed : 0.0;
       J.zf = h->p.flux_plane;
       J.gacc = gacc;
       J.cgrad = cgrad;
@@ -1326,6 +1375,13 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
       for (int q = 0; q < 5; q++) grad[q] = 0.0;
       for (int b = 0; b < nslot; b++)
         for (int q = 0; q < 5; q++) grad[q] += part[(size_t)b * 5 + q];
+      if (perm && W_ > 0) {
+        // the permeability's own dependence on tau (through nu) and on g_z
+        // (u_phys = u_eq - (tau - 1/2) g, and the driving gradient)
+        const double Jw = h->obj.Jwin, G = perm_drive(h), rA = h->p.rho_A[0];
+        grad[3] += Jw / (3.0 * nu);
+        grad[2] += rA * nu * (double)h->nf / (2.0 * G * (double)h->N) - rA * Jw / G;
+      }
       if (grad_cmap) {
         if ((e = hipMemcpy(grad_cmap, cgrad, (size_t)h->E * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
           fail("inlet gradient read-back", e);

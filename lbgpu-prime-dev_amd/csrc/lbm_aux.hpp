@@ -205,6 +205,23 @@ __global__ __launch_bounds__(64) void k_plane_flux_final(const double *partial, 
   }
 }
 
+// acc[0] += sum of x[0..n), in a fixed order: per-block partial sums, then
+// one thread adds them in block order
+__global__ __launch_bounds__(256) void k_sum_partial(const double *x, long n, double *partial) {
+  double v[1] = {0.0};
+  const int kind[1] = {0};
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) v[0] += x[i];
+  block_reduce<1>(v, kind, partial + blockIdx.x);
+}
+
+__global__ __launch_bounds__(64) void k_sum_final(const double *partial, int nb, double *acc) {
+  if (threadIdx.x == 0) {
+    double a = 0.0;
+    for (int q = 0; q < nb; q++) a += partial[q];
+    acc[0] += a;
+  }
+}
+
 // plain copy of n doubles: the bandwidth (and HBM byte counter) reference
 __global__ __launch_bounds__(256) void k_copy_probe(const double *__restrict__ src, double *__restrict__ dst,
                                                     long n) {

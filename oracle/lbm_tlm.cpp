@@ -50,6 +50,7 @@ inline double wt(int a) { return a == 0 ? 1. / 3. : ((a < 5 || a == 9 || a == 10
 
 struct Setup {
   int ni, nj, nk, nsteps, window, zf;
+  int kind;  // 0: solute flux through plane zf; 1: permeability
   const uint8_t *mask;
   double rho_bot, rho_top;
   // observation points: J += w (c(node, step) - target)^2
@@ -72,7 +73,9 @@ T run(const Setup &S, T tau, T taus, const T *g, const std::vector<T> &cmap) {
     return (long)i + (long)ni * j + E * k;
   };
   const T one(1.0), om = one / tau, oms = one / taus;
-  T flux(0.0), obs(0.0);
+  T flux(0.0), obs(0.0), jz(0.0);
+  long nfluid = 0;
+  for (long n = 0; n < N; n++) nfluid += !solid(n);
   for (int t = 1; t <= S.nsteps; t++) {
     // stream with halfway bounce-back
     for (int k = 0; k < nk; k++)
@@ -139,8 +142,16 @@ T run(const Setup &S, T tau, T taus, const T *g, const std::vector<T> &cmap) {
       const T r = c - T(S.otarget[q]);
       obs += T(S.oweight ? S.oweight[q] : 1.0) * r * r;
     }
-    // objective: solute flux through plane zf over the last `window` steps
-    if (S.window > 0 && t > S.nsteps - S.window) {
+    // objective: solute flux through plane zf, or the flow's z momentum
+    // over all nodes, over the last `window` steps
+    if (S.window > 0 && t > S.nsteps - S.window && S.kind == 1) {
+      for (long n = 0; n < N; n++) {
+        if (solid(n)) continue;
+        for (int a = 0; a < 19; a++)
+          if (CZ[a]) jz += T((double)CZ[a]) * F[a * N + n];
+      }
+    }
+    if (S.window > 0 && t > S.nsteps - S.window && S.kind == 0) {
       for (int j = 0; j < nj; j++)
         for (int i = 0; i < ni; i++) {
           long n = idx(i, j, S.zf);
@@ -173,6 +184,16 @@ T run(const Setup &S, T tau, T taus, const T *g, const std::vector<T> &cmap) {
       }
     }
   }
+  if (S.window > 0 && S.kind == 1) {
+    // permeability k = rho_A nu <u_z,phys> / G, rho_A = 1: nu = (tau - 1/2)/3,
+    // u_phys = j/rho_A + g/2 (superficial average over all N nodes), G the
+    // driving gradient (rho_bot - rho_top)/(3 (nk - 1)) + rho_A g_z
+    const T nu = (tau - T(0.5)) / T(3.0);
+    const T G = T((S.rho_bot - S.rho_top) / (3.0 * (nk - 1))) + g[2];
+    const T W((double)S.window);
+    const T uphys = (jz + W * T((double)nfluid) * g[2] / T(2.0)) / (W * T((double)N));
+    return nu * uphys / G + obs;
+  }
   if (S.window > 0) return flux / T((double)S.window * (double)E) + obs;
   return obs;
 }
@@ -187,13 +208,16 @@ extern "C" {
 // dtau_s}; cmap, dcmap: [nj][ni].  J is the flux objective over the last
 // `window` steps (none for window 0) plus the misfit of the nobs
 // observation points (dense node, step 1..nsteps, target, weight or NULL).
-int orc_tlm_objective_obs(int ni, int nj, int nk, const uint8_t *mask, double rho_bot, double rho_top, int zf,
-                          int nsteps, int window, const double *params, const double *cmap, int nobs,
-                          const long *onode, const int *ostep, const double *otarget, const double *oweight,
-                          const double *dparams, const double *dcmap, double *J, double *dJ) {
+// kind 0: the solute flux through plane zf; kind 1: the permeability
+// rho_A nu <u_z,phys> / G averaged over the window (zf unused).
+int orc_tlm_objective_kind(int kind, int ni, int nj, int nk, const uint8_t *mask, double rho_bot, double rho_top,
+                           int zf, int nsteps, int window, const double *params, const double *cmap, int nobs,
+                           const long *onode, const int *ostep, const double *otarget, const double *oweight,
+                           const double *dparams, const double *dcmap, double *J, double *dJ) {
   if (ni <= 0 || nj <= 0 || nk <= 2 || !mask || !params || !cmap || window < 0 || window > nsteps) return 1;
   if (nobs < 0 || (window == 0 && nobs == 0) || (nobs && (!onode || !ostep || !otarget))) return 1;
-  Setup S{ni, nj, nk, nsteps, window, zf, mask, rho_bot, rho_top, nobs, onode, ostep, otarget, oweight};
+  if (kind != 0 && kind != 1) return 1;
+  Setup S{ni, nj, nk, nsteps, window, zf, kind, mask, rho_bot, rho_top, nobs, onode, ostep, otarget, oweight};
   const long E = (long)ni * nj;
   if (!dparams) {
     std::vector<double> cm(cmap, cmap + E);
@@ -209,6 +233,14 @@ int orc_tlm_objective_obs(int ni, int nj, int nk, const uint8_t *mask, double rh
   *J = val(r);
   if (dJ) *dJ = der(r);
   return 0;
+}
+
+int orc_tlm_objective_obs(int ni, int nj, int nk, const uint8_t *mask, double rho_bot, double rho_top, int zf,
+                          int nsteps, int window, const double *params, const double *cmap, int nobs,
+                          const long *onode, const int *ostep, const double *otarget, const double *oweight,
+                          const double *dparams, const double *dcmap, double *J, double *dJ) {
+  return orc_tlm_objective_kind(0, ni, nj, nk, mask, rho_bot, rho_top, zf, nsteps, window, params, cmap, nobs,
+                                onode, ostep, otarget, oweight, dparams, dcmap, J, dJ);
 }
 
 // the flux objective alone

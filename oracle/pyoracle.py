@@ -84,6 +84,8 @@ def lib():
                                             ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_int), dp, dp,
                                             dp, dp, dp, dp]
+        L.orc_tlm_objective_kind.restype = ctypes.c_int
+        L.orc_tlm_objective_kind.argtypes = [ctypes.c_int] + L.orc_tlm_objective_obs.argtypes
         L.orc_set_ns.restype = ctypes.c_int
         L.orc_set_ns.argtypes = [vp, dp]
         L.orc_num_threads.restype = ctypes.c_int
@@ -276,10 +278,13 @@ def num_threads():
 
 
 def tlm_objective(mask, rho_bot, rho_top, zf, nsteps, window, params, cmap, dparams=None, dcmap=None,
-                  observations=None):
-    """(J, dJ) of the solute-flux objective (window > 0) plus the misfit of
-    observations = (dense node, step, target, weight or None) from the FP64
-    tangent-linear model (oracle/lbm_tlm.cpp); params = (gx, gy, gz, tau, tau_s)."""
+                  observations=None, objective="flux"):
+    """(J, dJ) of the window objective (window > 0: the solute flux through
+    plane zf, or with objective="permeability" the permeability
+    rho_A nu <u_z,phys> / G) plus the misfit of observations = (dense node,
+    step, target, weight or None) from the FP64 tangent-linear model
+    (oracle/lbm_tlm.cpp); params = (gx, gy, gz, tau, tau_s)."""
+    kind = {"flux": 0, "permeability": 1}[objective]
     m = np.ascontiguousarray(mask, dtype=np.uint8)
     nk, nj, ni = m.shape
     p = np.ascontiguousarray(list(params) + [0.0], dtype=np.float64)
@@ -294,12 +299,12 @@ def tlm_objective(mask, rho_bot, rho_top, zf, nsteps, window, params, cmap, dpar
         os_ = np.ascontiguousarray(observations[1], dtype=np.int32)
         ot = np.ascontiguousarray(observations[2], dtype=np.float64)
         ow = None if observations[3] is None else np.ascontiguousarray(observations[3], dtype=np.float64)
-    rc = lib().orc_tlm_objective_obs(
-        ni, nj, nk, m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), rho_bot, rho_top, zf, nsteps, window,
+    rc = lib().orc_tlm_objective_kind(
+        kind, ni, nj, nk, m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), rho_bot, rho_top, zf, nsteps, window,
         _dp(p), _dp(c), on.size, on.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
         os_.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(ot), _dp(ow) if ow is not None else None,
         _dp(dpp) if dpp is not None else None, _dp(dcc) if dcc is not None else None, ctypes.byref(J),
         ctypes.byref(dJ))
     if rc:
-        raise ValueError("orc_tlm_objective_obs: bad arguments")
+        raise ValueError("orc_tlm_objective_kind: bad arguments")
     return J.value, dJ.value

@@ -175,3 +175,48 @@ def test_adjoint_rejects_unsupported_configurations():
     lat = L.Lattice(mask=mask, dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(0.8, 0.75), flux_plane=nk - 2)
     with pytest.raises(L.LbmError):
         lat.adjoint()
+
+
+def test_permeability_adjoint_matches_tlm_and_differences():
+    """the permeability objective: the adjoint gradient against the FP64
+    TLM (1e-12) and central differences of the GPU forward run (1e-5); the
+    flow does not see the solute, so the tau_s and inlet-map gradients are
+    exactly zero"""
+    say("case permeability adjoint 10x8x16, 60 steps, window 20, checkpoints every 25")
+    L, mask, cmap = setup()
+    nk, nj, ni = mask.shape
+
+    def objective(params):
+        lat = L.Lattice(mask=mask, dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(params[3], params[4]),
+                        gaccel=tuple(params[:3]) + (0.0, 0.0, 0.0), zface_pressure=1, rho_bot=1.001,
+                        rho_top=1.0, zface_solute=1, c_bot_map=cmap, flux_plane=nk - 2)
+        lat.init_rest()
+        return lat, lat.objective(60, 20, 25, kind="permeability")
+
+    lat, J = objective(PARAMS)
+    grad, gmap = lat.adjoint()
+    J_tlm, _ = tlm_objective(mask, 1.001, 1.0, 0, 60, 20, PARAMS, cmap, objective="permeability")
+    say(f"J gpu {J:.15e} tlm {J_tlm:.15e}")
+    assert J > 0 and abs(J - J_tlm) <= 1e-12 * abs(J_tlm)
+    for q in range(5):
+        dp = np.zeros(5)
+        dp[q] = 1.0
+        _, dJ = tlm_objective(mask, 1.001, 1.0, 0, 60, 20, PARAMS, cmap, dp, np.zeros_like(cmap),
+                              objective="permeability")
+        say(f"param {q}: adjoint {grad[q]:.12e} tlm {dJ:.12e}")
+        # dJ/dg_z is the small difference of the explicit terms (about 2e4)
+        # and the dynamics' part, so its relative round-off is ~40x larger
+        assert abs(grad[q] - dJ) <= 1e-11 * abs(dJ) + 1e-22
+    assert grad[4] == 0.0 and not gmap.any()
+    for q, h in ((2, 1e-8), (3, 1e-6)):
+        dp = np.zeros(5)
+        dp[q] = h
+        _, Jp = objective(PARAMS + dp)
+        _, Jm = objective(PARAMS - dp)
+        fd = (Jp - Jm) / (2 * h)
+        say(f"param {q}: adjoint {grad[q]:.10e} fd {fd:.10e}")
+        assert abs(grad[q] - fd) <= 1e-5 * abs(fd)
+    # the objective kind is part of the handle's state: back to the flux
+    _, Jf = gpu_objective(L, mask, cmap, PARAMS, 60, 20, 25)
+    J_flux, _ = tlm_objective(mask, 1.001, 1.0, nk - 2, 60, 20, PARAMS, cmap)
+    assert abs(Jf - J_flux) <= 1e-12 * abs(J_flux)

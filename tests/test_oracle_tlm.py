@@ -118,3 +118,60 @@ def test_tlm_observation_derivative_equals_central_difference(which):
     fd = (Jp - Jm) / (2 * h)
     assert dJ != 0.0
     assert abs(dJ - fd) <= 1e-6 * abs(dJ)
+
+
+def oracle_permeability(mask, cmap, params, nsteps, window, rho_bot=1.001, rho_top=1.0):
+    """rho_A nu <u_z,phys> / G over the window from the AA oracle's
+    macroscopic writes: u_phys = u_eq - (tau - 1/2) g (superficial average),
+    G = (rho_bot - rho_top)/(3 (LZ - 1)) + rho_A g_z, rho_A = 1"""
+    nk, nj, ni = mask.shape
+    o = Oracle(dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(params[3], params[4]),
+               gaccel=tuple(params[:3]) + (0.0, 0.0, 0.0), zface_pressure=1, rho_bot=rho_bot, rho_top=rho_top,
+               zface_solute=1, c_bot_map=cmap, flux_plane=nk - 2)
+    o.set_solids(mask)
+    o.init_rest()
+    fl = mask.reshape(-1) != 1
+    acc = 0.0
+    for t in range(1, nsteps + 1):
+        o.step(t > nsteps - window)
+        if t > nsteps - window:
+            acc += np.sum(o.mv(0)[3][fl] - (params[3] - 0.5) * params[2])
+    nu = (params[3] - 0.5) / 3
+    G = (rho_bot - rho_top) / (3 * (nk - 1)) + params[2]
+    return nu * acc / (window * mask.size) / G
+
+
+def test_tlm_permeability_equals_oracle():
+    mask, cmap = case()
+    params = (0.0, 0.0, 1e-5, 0.8, 0.75)
+    J_tlm, _ = tlm_objective(mask, 1.001, 1.0, 0, 60, 20, params, cmap, objective="permeability")
+    J_orc = oracle_permeability(mask, cmap, params, 60, 20)
+    assert J_orc > 0.0
+    assert abs(J_tlm - J_orc) <= 1e-12 * abs(J_orc)
+
+
+@pytest.mark.parametrize("which", range(6))
+def test_tlm_permeability_derivative_equals_central_difference(which):
+    """includes the explicit dependence on tau (through nu) and on g_z
+    (through u_phys and the driving gradient)"""
+    mask, cmap = case()
+    params = np.array([0.0, 0.0, 1e-5, 0.8, 0.75])
+    rng = np.random.default_rng(8)
+    dp = np.zeros(5)
+    dc = np.zeros_like(cmap)
+    if which < 5:
+        dp[which] = 1.0
+        h = 1e-8 if which < 3 else 1e-6
+    else:
+        dc = rng.uniform(-1, 1, cmap.shape)
+        h = 1e-5
+    kw = dict(objective="permeability")
+    _, dJ = tlm_objective(mask, 1.001, 1.0, 0, 50, 15, params, cmap, dp, dc, **kw)
+    Jp, _ = tlm_objective(mask, 1.001, 1.0, 0, 50, 15, params + h * dp, cmap + h * dc, **kw)
+    Jm, _ = tlm_objective(mask, 1.001, 1.0, 0, 50, 15, params - h * dp, cmap - h * dc, **kw)
+    fd = (Jp - Jm) / (2 * h)
+    if which >= 4:
+        # the flow does not see the solute: no dependence on tau_s or the inlet map
+        assert dJ == 0.0 and fd == 0.0
+    else:
+        assert abs(dJ - fd) <= 1e-6 * abs(fd) + 1e-12, (dJ, fd)
