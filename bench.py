@@ -165,13 +165,16 @@ def precision_table(L, n=192, steps=4000):
         rho, ux, uy, uz = lat.mv(0)
         c = lat.mv(1)[0]
         flux = lat.last_flux()[0]
-        k = permeability(lat.reduce_sums(), n ** 3, 0.8, 1.001, 1.0, n - 1)
-        runs[name] = (rho[fl] - 1.0, np.stack([ux[fl], uy[fl], uz[fl]]), c[fl], flux, k, wall)
+        sums = lat.reduce_sums()
+        k = permeability(sums, n ** 3, 0.8, 1.001, 1.0, n - 1)
+        # the FP64 GPU reductions: flow mass sum(rho - rho_A), solute mass sum(c)
+        runs[name] = (rho[fl] - 1.0, np.stack([ux[fl], uy[fl], uz[fl]]), c[fl], flux, k, wall, sums[0], sums[5])
         lat.close()
         log(f"[precision] {name} {wall:.2f}s flux {flux:.6e} k {k:.6e}")
     ref = runs["fp64"]
     out = {"lattice": [n, n, n], "porosity": round(phi, 5), "steps": steps,
-           "permeability_fp64": ref[4], "outlet_flux_fp64": ref[3]}
+           "permeability_fp64": ref[4], "outlet_flux_fp64": ref[3], "flow_mass_fp64": ref[6],
+           "solute_mass_fp64": ref[7]}
 
     def rel(a, b):
         return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
@@ -180,6 +183,7 @@ def precision_table(L, n=192, steps=4000):
             continue
         out[name] = {"drho": rel(r[0], ref[0]), "u": rel(r[1], ref[1]), "c": rel(r[2], ref[2]),
                      "outlet_flux": abs(r[3] - ref[3]) / abs(ref[3]), "permeability": abs(r[4] - ref[4]) / abs(ref[4]),
+                     "flow_mass": abs(r[6] - ref[6]) / abs(ref[6]), "solute_mass": abs(r[7] - ref[7]) / abs(ref[7]),
                      "seconds": round(r[5], 3)}
     return out
 

@@ -508,6 +508,8 @@ extern "C" int lbm_create(const lbm_params *p, lbm_handle **out) {
     return lbm_fail("FP16 storage runs the FP32 collision");
   long N = (long)p->ni * p->nj * (p->dims == 3 ? p->nk : 1);
   if (N >= (1L << 31) - (1L << 24)) return lbm_fail("lattice too large for 32-bit node indices");
+  if (p->flux_plane >= (p->dims == 2 ? p->nj : p->nk))
+    return lbm_fail("flux_plane %d lies outside the lattice", p->flux_plane);
   lbm_handle *h = new lbm_handle();
   h->p = *p;
   if (p->dims == 2) h->p.nk = 1;

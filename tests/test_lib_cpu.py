@@ -92,3 +92,26 @@ def test_benchmark_geometries_recorded(name, args, sha, phi):
     m, p = L.spheres3d(*args, zbuf=4)
     assert hashlib.sha256(m.tobytes()).hexdigest() == sha, name
     assert p == phi
+
+
+@pytest.mark.parametrize("bad,match", [
+    (dict(dims=4), "dims must be 2 or 3"),
+    (dict(ni=0), "bad lattice size"),
+    (dict(dims=3, nk=0), "bad lattice size"),
+    (dict(nsubs=3), "nsubs must be 1 or 2"),
+    (dict(tau=(0.5, 1.0)), "must exceed 1/2"),
+    (dict(nsubs=2, tau=(0.8, 0.4)), "must exceed 1/2"),
+    (dict(storage=3), "bad storage/compute"),
+    (dict(storage=2, compute=0), "FP16 storage runs the FP32 collision"),
+    (dict(dims=3, ni=2048, nj=1024, nk=1024), "too large for 32-bit node indices"),
+    (dict(dims=3, nk=5, flux_plane=5), "flux_plane 5 lies outside"),
+    (dict(flux_plane=6), "flux_plane 6 lies outside"),
+])
+def test_create_rejects_bad_parameters(bad, match):
+    """lbm_create checks its parameters before it touches the GPU, and
+    reports the reason (the reference's process_exit(1) on bad input)"""
+    L = _pkg.load()
+    kw = dict(dims=2, ni=8, nj=6, nk=1, tau=(0.8, 0.8))
+    kw.update(bad)
+    with pytest.raises(L.LbmError, match=match):
+        L.Lattice(**kw)
