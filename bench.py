@@ -237,22 +237,38 @@ def slab_c5(L, n=448, steps=2000):
     return res
 
 
-def cpu_baseline(mask, kw, budget_s):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(mask, kw, budget_s, repeats=3):
+    """the CPU oracle on the same lattice: `repeats` timed samples of whole
+    step pairs, each about budget_s / repeats long; returns (steps, seconds)
+    of each sample and the thread count"""
     sys.path.insert(0, ROOT)
     from oracle import pyoracle
     o = pyoracle.Oracle(**kw)
     o.set_solids(mask)
     o.init_rest()
     o.run(2, write_last=False)
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        o.run(2, write_last=False)
-        n += 2
-        el = time.perf_counter() - t0
-        if el >= budget_s:
-            break
-    return n, el, pyoracle.num_threads()
+    samples = []
+    for r in range(repeats):
+        n = 0
+        t0 = time.perf_counter()
+        while True:
+            o.run(2, write_last=False)
+            n += 2
+            el = time.perf_counter() - t0
+            if el >= budget_s / repeats:
+                break
+        samples.append((n, el))
+    return samples, pyoracle.num_threads()
 
 
 def main():
@@ -385,11 +401,16 @@ def main():
             result["extra"]["slab_c5"] = slab_c5(L)
 
     if not args.no_cpu:
-        n_cpu, el, threads = cpu_baseline(mask, dict(base, nsubs=1), args.cpu_seconds)
+        samples, threads = cpu_baseline(mask, dict(base, nsubs=1), args.cpu_seconds)
+        rates = sorted(N * n_cpu / el / 1e6 for n_cpu, el in samples)
+        med = rates[len(rates) // 2]
         result["cpu_baseline"] = {
-            "value": round(N * n_cpu / el / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"CPU oracle (C, OpenMP {threads} threads), same {n}^3 flow FP64 lattice, "
-                      f"{n_cpu} steps after 2 warm-up steps, {el:.1f} s",
+            "value": round(med, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (C, OpenMP, {threads} threads on {os.cpu_count()} visible CPUs, "
+                      f"{cpu_model()}, OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}), "
+                      f"same {n}^3 flow FP64 lattice; median of {len(samples)} samples of "
+                      + ", ".join(f"{n_cpu} steps in {el:.1f} s" for n_cpu, el in samples),
+            "samples_mlups": [round(r, 2) for r in rates],
         }
     print(json.dumps(result), flush=True)
 

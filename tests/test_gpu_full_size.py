@@ -10,13 +10,17 @@ configurations, where the FP64 oracle would take minutes:
   bit for bit;
 - 128^3 flow + solute (porosity 0.40), 2000 steps with checkpoints every
   100: the discrete-adjoint gradient equals central differences of the
-  forward run to 1e-5."""
+  forward run to 1e-5;
+and, at reduced step counts, parity with the FP64 oracle on the 256^3 flow
+and 192^3 flow + solute configurations' own seeded geometry and boundary
+conditions."""
 import sys
 
 import numpy as np
 import pytest
 
 import _pkg
+from test_gpu_parity import build_pair, compare
 
 pytestmark = pytest.mark.gpu
 
@@ -134,3 +138,29 @@ def test_c4_adjoint_matches_central_differences_full_size():
     adj = float(np.sum(gmap * d))
     say(f"inlet map direction: adjoint {adj:.10e} fd {fd:.10e}")
     assert abs(adj - fd) <= 1e-5 * abs(fd)
+
+
+@pytest.mark.parametrize("case", ["256^3 flow", "192^3 flow + solute"])
+def test_parity_on_benchmark_geometry(case):
+    """the GPU path against the FP64 oracle on the benchmark configurations'
+    seeded sphere packs, z-face conditions and sizes, 20 steps (a few
+    seconds of oracle time); bit for bit"""
+    say(f"case {case} geometry, 20 steps against the oracle")
+    L = _pkg.load()
+    if case.startswith("256"):
+        n, nsubs = 256, 1
+        mask, _ = L.spheres3d(n, n, n, 202, 6.0, 14.0, 0.40, zbuf=4)
+        kw = dict(dims=3, ni=n, nj=n, nk=n, tau=(0.8, 0.75), zface_pressure=1, rho_bot=1.001, rho_top=1.0)
+    else:
+        n, nsubs = 192, 2
+        mask, _ = L.spheres3d(n, n, n, 303, 6.0, 14.0, 0.45, zbuf=4)
+        kw = dict(dims=3, ni=n, nj=n, nk=n, nsubs=2, tau=(0.8, 0.75), zface_pressure=1, rho_bot=1.001,
+                  rho_top=1.0, zface_solute=1, c_bot=1.0, flux_plane=n - 2)
+    o, g = build_pair(mask, **kw)
+    o.init_rest()
+    g.init_rest()
+    o.run(20)
+    g.run(20)
+    _, bitwise = compare(o, g, mask, nsubs, f"{case}, 20 steps")
+    assert bitwise
+    g.close()
