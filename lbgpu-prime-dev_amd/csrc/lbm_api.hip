@@ -372,7 +372,7 @@ int launch_bcs(lbm_handle *h, int phase, int t) {
           B.list = c.north ? h->face_hi : h->face_lo;
           B.n = c.north ? h->n_hi : h->n_lo;
           if (!B.n) continue;
-          CT rb = (CT)bc_rho(h, s, c.which, t);
+          const double rb = bc_rho(h, s, c.which, t);
           if (s == 0)
             hipLaunchKernelGGL((k_bc2d_pressure<NS, ST, CT, 0>), dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B,
                                c.north, rb);

@@ -238,6 +238,8 @@ __device__ __forceinline__ void collide(const StepArgs<ST, CT> &A, int idx, CT (
   CT rho = f[0];
 #pragma unroll
   for (int a = 1; a < Q; a++) rho += f[a];
+  const CT rho_pre = rho;  // mass before the sink
+  CT jpre[3] = {(CT)0, (CT)0, (CT)0};  // momentum before the collision (FP32 correction)
   if (SUBS == 0) {
     u[0] = (CT)0;
     u[1] = (CT)0;
@@ -248,6 +250,19 @@ __device__ __forceinline__ void collide(const StepArgs<ST, CT> &A, int idx, CT (
       u[1] += ((CT)CY(a)) * f[a];
       if (D == 3) u[2] += ((CT)CZ(a)) * f[a];
     }
+    if constexpr (sizeof(CT) == 4 && sizeof(ST) >= 4) {
+#pragma unroll
+      for (int d = 0; d < D; d++) jpre[d] = u[d];
+    }
+  } else if constexpr (sizeof(CT) == 4 && sizeof(ST) >= 4) {
+#pragma unroll
+    for (int a = 1; a < Q; a++) {
+      jpre[0] += ((CT)CX(a)) * f[a];
+      jpre[1] += ((CT)CY(a)) * f[a];
+      if (D == 3) jpre[2] += ((CT)CZ(a)) * f[a];
+    }
+  }
+  if (SUBS == 0) {
     if constexpr (WALSH) {
 #pragma unroll
       for (int d = 0; d < D; d++) u[d] *= kns;
@@ -272,7 +287,7 @@ __device__ __forceinline__ void collide(const StepArgs<ST, CT> &A, int idx, CT (
     const bool part = A.write_mv == 2;
     if (!part || SUBS == 1) {
       if (SUBS == 0)
-        mv[idx] = (double)(rho + A.rhoA);
+        mv[idx] = (double)rho + (double)A.rhoA;  // in FP64 before the sum (FP32: rho is an offset)
       else
         mv[idx] = (double)rho;
     }
@@ -300,6 +315,44 @@ __device__ __forceinline__ void collide(const StepArgs<ST, CT> &A, int idx, CT (
       fa += w * A.rhoA * ((CT)3 * vdotu + (CT)4.5 * vdotu * vdotu - (CT)1.5 * usq) * it;
     }
     f[a] = fa;
+  }
+  if constexpr (sizeof(CT) == 4 && sizeof(ST) >= 4) {
+    // FP32 collision (FP32 or FP64 storage; FP16 storage, whose rounding
+    // is far larger, keeps the plain update and its speed).  The weights
+    // rounded to FP32 sum to 1 + 1.5e-8 and
+    // give sum_a w_a c_a,x^2 = 1/3 + 2.5e-9, so every collision would add
+    // mass and momentum of that relative size: a drift of ~1e-4 in the
+    // density and ~5e-5 in the permeability over 4000 steps.  The
+    // post-collision moments are set to their exact values instead, with
+    // unbiased rounding: the momentum (1 - w) j + w rho_A u (flow) or
+    // (1 - w_s) j + w_s c u (solute) through the pair of axis directions
+    // of each component, then the mass (flow delta-rho, solute (1 - w_s) c
+    // + w_s c after the sink) through the rest population.
+    {
+      const CT m1 = SUBS == 0 ? A.rhoA : rho;
+      CT jt[3];
+#pragma unroll
+      for (int d = 0; d < D; d++) jt[d] = ((CT)1 - it) * jpre[d] + it * m1 * u[d];
+      constexpr int pos[3] = {1, 3, 9}, neg[3] = {2, 4, 10};
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        CT jc = (CT)0;
+#pragma unroll
+        for (int a = 1; a < Q; a++) {
+          const int c = d == 0 ? CX(a) : (d == 1 ? CY(a) : CZ(a));
+          if (c > 0) jc += f[a];
+          if (c < 0) jc -= f[a];
+        }
+        const CT half = (CT)0.5 * (jt[d] - jc);
+        f[pos[d]] += half;
+        f[neg[d]] -= half;
+      }
+    }
+    const CT mass = SUBS == 0 ? rho_pre : ((CT)1 - it) * rho_pre + it * rho;
+    CT others = f[1];
+#pragma unroll
+    for (int a = 2; a < Q; a++) others += f[a];
+    f[0] = mass - others;
   }
 }
 
@@ -395,7 +448,8 @@ __device__ __forceinline__ void zface_fix(const StepArgs<ST, CT> &A, int idx, CT
   const bool top = idx >= A.zb.hi_begin;
   CT v[5];
   if (A.zb.flow) {
-    zface_flow_values<CT>(f, top, (CT)(top ? A.zb.rho_top : A.zb.rho_bot) - A.rhoA, v);
+    // the offset rho_b - rho_A formed in FP64 (the same expression in FP64)
+    zface_flow_values<CT>(f, top, (CT)((top ? A.zb.rho_top : A.zb.rho_bot) - (double)A.rhoA), v);
     zface_put<ST, CT, 0>(A, top, v, f);
   }
   if (SOLUTE && A.zb.solute) {
@@ -511,8 +565,11 @@ __device__ __forceinline__ void bc_store(const BcArgs<ST, CT> &B, int idx, const
 
 // 2-D Zou-He pressure on the N (j = nj-1) or S (j = 0) row:
 // k_sysbound_pressure_{n,s}_{1,2}, src/system_boundary_kernels.c:4-451
+// The prescribed density arrives in FP64: in the FP32 collision the
+// offset rho_b - rho_A is formed in FP64 first (1.001 rounded to FP32 would
+// move a 1e-3 offset by 5e-5), in FP64 the reference's order is kept.
 template <int NS, typename ST, typename CT, int SUBS>
-__global__ void k_bc2d_pressure(BcArgs<ST, CT> B, int north, CT rho_b) {
+__global__ void k_bc2d_pressure(BcArgs<ST, CT> B, int north, double rho_b) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= B.n) return;
   const int idx = B.list[t];
@@ -523,8 +580,12 @@ __global__ void k_bc2d_pressure(BcArgs<ST, CT> B, int north, CT rho_b) {
   const CT rhoA = SUBS == 0 ? B.A.rhoA : B.A.rhoA_s;
   if (north) {
     CT x = f[C_] + f[E_] + f[W_] + f[N_] + f[N_] + f[NE_] + f[NE_] + f[NW_] + f[NW_];
-    x -= rho_b;
-    x += rhoA;
+    if constexpr (sizeof(CT) == 4) {
+      x -= (CT)(rho_b - (double)rhoA);
+    } else {
+      x -= (CT)rho_b;
+      x += rhoA;
+    }
     CT vS = f[N_] - ((CT)(2. / 3.)) * x;
     CT vSW = f[NE_] + ((CT)(1. / 2.)) * (f[E_] - f[W_]) - ((CT)(1. / 6.)) * x;
     CT vSE = f[NW_] + ((CT)(1. / 2.)) * (f[W_] - f[E_]) - ((CT)(1. / 6.)) * x;
@@ -533,8 +594,12 @@ __global__ void k_bc2d_pressure(BcArgs<ST, CT> B, int north, CT rho_b) {
     bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, SW_, vSW, false);
   } else {
     CT x = -f[C_] - f[E_] - f[W_] - f[S_] - f[S_] - f[SE_] - f[SE_] - f[SW_] - f[SW_];
-    x += rho_b;
-    x -= rhoA;
+    if constexpr (sizeof(CT) == 4) {
+      x += (CT)(rho_b - (double)rhoA);
+    } else {
+      x += (CT)rho_b;
+      x -= rhoA;
+    }
     CT vN = f[S_] + ((CT)(2. / 3.)) * x;
     CT vNE = f[SW_] + ((CT)(1. / 2.)) * (f[W_] - f[E_]) + ((CT)(1. / 6.)) * x;
     CT vNW = f[SE_] + ((CT)(1. / 2.)) * (f[E_] - f[W_]) + ((CT)(1. / 6.)) * x;
@@ -592,7 +657,7 @@ __global__ void k_bc_zface(BcArgs<ST, CT> B, ZfaceArgs Z) {
   const bool top = Z.top;
   if (Z.flow) {
     bc_gather<19, NS, ST, CT, 0>(B, idx, nb, f);
-    zface_flow_values<CT>(f, top, (CT)Z.rho_b - B.A.rhoA, v);
+    zface_flow_values<CT>(f, top, (CT)(Z.rho_b - (double)B.A.rhoA), v);
     if (top) {
 #pragma unroll
       for (int q = 0; q < 5; q++) bc_store<19, NS, ST, CT, 0>(B, idx, nb, kDn[q], v[q], true);

@@ -327,7 +327,7 @@ def test_halo_pack_buffers_match_oracle():
     compare(o, g, mask, 2, "explicit halo buffers, 2 steps")
 
 
-@pytest.mark.parametrize("storage,compute,tol", [(0, 1, 1e-3), (1, 1, 1e-3), (1, 0, 1e-4), (2, 1, 1e-1)])
+@pytest.mark.parametrize("storage,compute,tol", [(0, 1, 1e-5), (1, 1, 1e-5), (1, 0, 1e-6), (2, 1, 1e-2)])
 def test_reduced_precision_error_vs_fp64(storage, compute, tol):
     """storage FP64/FP32/FP16 with FP32 collision against the FP64 oracle
     (bounds are stated error levels, not parity)"""
@@ -352,3 +352,28 @@ def test_reduced_precision_error_vs_fp64(storage, compute, tol):
     ec = relnorm(g.mv(1)[0][fl], o.mv(1)[0][fl])
     say(f"storage={storage} compute={compute}: drho err {er:.2e} u err {eu:.2e} c err {ec:.2e}")
     assert eu <= tol and ec <= tol
+
+
+def test_fp32_collision_permeability_within_1e_6():
+    """FP32-shifted storage with the FP32 collision: the permeability of a
+    pressure-driven sphere pack stays within 1e-6 of the FP64 run (the
+    collision closes its mass and momentum exactly, and the boundary density
+    offsets are formed in FP64)"""
+    say("case FP32 collision permeability")
+    L = _pkg.load()
+    n, steps = 48, 1500
+    mask, _ = L.spheres3d(n, n, n, 77, 3.0, 6.0, 0.45, zbuf=4)
+    kw = dict(dims=3, ni=n, nj=n, nk=n, nsubs=2, tau=(0.8, 0.75), zface_pressure=1, rho_bot=1.001,
+              rho_top=1.0, zface_solute=1, c_bot=1.0)
+    k = {}
+    for st, ct in ((0, 0), (1, 1)):
+        g = L.Lattice(mask=mask, storage=st, compute=ct, **kw)
+        g.init_rest()
+        g.run(steps)
+        s = g.reduce_sums()
+        nu = (0.8 - 0.5) / 3
+        k[st] = nu * (s[3] / n ** 3) / ((1.001 - 1.0) / 3 / (n - 1))
+        g.close()
+    err = abs(k[1] - k[0]) / abs(k[0])
+    say(f"permeability FP64 {k[0]:.10e} FP32 {k[1]:.10e} relative {err:.2e}")
+    assert err <= 1e-6
