@@ -251,30 +251,6 @@ __device__ __forceinline__ void adj_block_sum(const AdjArgs &J, long slot, doubl
   if (threadIdx.x < 5) J.gacc[slot * 5 + threadIdx.x] += red[threadIdx.x];
 }
 
-// Adjoint of one half-step at every fluid node, or, with A.zb.on, at every
-// node off the two z faces (k_adjoint_face takes those).
-template <bool ODD, bool OFF32>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ODD ? LBM_ADJ_WAVES_ODD : LBM_ADJ_WAVES_EVEN)))
-void k_adjoint_step(AdjArgs J) {
-  const int lb = xcd_block(J.A.nblk8);
-  const int idx = lb * kBlock + (int)threadIdx.x;
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  if (idx < J.A.nf && !zface_node<19>(J.A, idx)) {
-    int nb[19], k;
-    adj_neighbours<ODD>(J.A, idx, nb, k);
-    double lam[19], mu[19];
-    AdjMoments m;
-    adj_load<ODD, OFF32>(J, nb, idx, lam, mu, m);
-    AdjState st;
-    adj_state_stream<0, 19, ODD, OFF32>(st, J, nb, idx, lam, mu);
-    const AdjOut o = adj_collide(J, m, st);
-    adj_store_all<0, ODD, OFF32>(J, o, k == J.zf ? J.seed : 0.0, J.obs ? J.obs[idx] : 0.0, nb, idx, lam, mu);
-#pragma unroll
-    for (int q = 0; q < 5; q++) acc[q] = o.acc[q];
-  }
-  adj_block_sum(J, lb, acc);
-}
-
 // transpose of the flow z-face condition on the new adjoint of a face node:
 // the unknowns' adjoint goes to the known populations it was formed from
 __device__ __forceinline__ void adj_zface_flow(bool top, double (&l)[19]) {
@@ -364,9 +340,9 @@ __device__ __forceinline__ double adj_zface_solute(bool top, double (&l)[19]) {
 // forward half-step formed them) and their transpose applied before the
 // store.  Thread t < lo_end takes compact node t, the others the top face.
 template <bool ODD, bool OFF32>
-__global__ __launch_bounds__(kBlock) void k_adjoint_face(AdjArgs J, int slot0) {
+__device__ __forceinline__ void adj_face_block(const AdjArgs &J, long slot, int fb) {
   const StepArgs<double, double> &A = J.A;
-  const int t = blockIdx.x * kBlock + (int)threadIdx.x;
+  const int t = fb * kBlock + (int)threadIdx.x;
   const int nlo = A.zb.lo_end, nface = nlo + (A.nf - A.zb.hi_begin);
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (t < nface) {
@@ -405,7 +381,38 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_face(AdjArgs J, int slot0) {
 #pragma unroll
     for (int q = 0; q < 5; q++) acc[q] = o.acc[q];
   }
-  adj_block_sum(J, slot0 + (long)blockIdx.x, acc);
+  adj_block_sum(J, slot, acc);
+}
+
+// Adjoint of one half-step.  With A.zb.on the first nfb blocks (a multiple
+// of 8, so the XCD mapping of the others is unchanged) take the z-face
+// nodes (adj_face_block) and the others every node off the faces; without
+// it (nfb = 0) every block takes the nodes of its range.  Gradient partials:
+// slot b of face block b, slot nfb + logical block of the others.
+template <bool ODD, bool OFF32>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ODD ? LBM_ADJ_WAVES_ODD : LBM_ADJ_WAVES_EVEN)))
+void k_adjoint_step(AdjArgs J, int nfb) {
+  if ((int)blockIdx.x < nfb) {
+    adj_face_block<ODD, OFF32>(J, blockIdx.x, blockIdx.x);
+    return;
+  }
+  const int lb = xcd_block(J.A.nblk8, (int)blockIdx.x - nfb);
+  const int idx = lb * kBlock + (int)threadIdx.x;
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (idx < J.A.nf && !zface_node<19>(J.A, idx)) {
+    int nb[19], k;
+    adj_neighbours<ODD>(J.A, idx, nb, k);
+    double lam[19], mu[19];
+    AdjMoments m;
+    adj_load<ODD, OFF32>(J, nb, idx, lam, mu, m);
+    AdjState st;
+    adj_state_stream<0, 19, ODD, OFF32>(st, J, nb, idx, lam, mu);
+    const AdjOut o = adj_collide(J, m, st);
+    adj_store_all<0, ODD, OFF32>(J, o, k == J.zf ? J.seed : 0.0, J.obs ? J.obs[idx] : 0.0, nb, idx, lam, mu);
+#pragma unroll
+    for (int q = 0; q < 5; q++) acc[q] = o.acc[q];
+  }
+  adj_block_sum(J, nfb + (long)lb, acc);
 }
 
 // Transpose of the z-face conditions of k_bc_zface for one face.

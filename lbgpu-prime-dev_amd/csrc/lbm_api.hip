@@ -196,7 +196,7 @@ StepArgs<ST, CT> make_args(const lbm_handle *h) {
 }
 
 // Whether the z-face conditions of step t can be formed inside the
-// half-step (k_half_step / k_adjoint_face) instead of a separate launch
+// half-step (k_half_step / the face blocks of k_adjoint_step) instead of a separate launch
 // storing them first.  Not when the plane flux is taken on a face (it reads
 // the stored boundary values) or when the solute condition would write into
 // a resting solute (the separate launch does, and later steps see it).
@@ -1223,7 +1223,8 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
   if (!rc && !h->obj.L && (e = hipMalloc(&h->obj.L, sb)) != hipSuccess) fail("adjoint state", e);
   // gradient partials: one slot per block of the step kernel, then one per
   // block of the face-node kernel
-  const int nfb = (int)blocks_for((long)h->n_lo + h->n_hi, kBlock);
+  // face blocks of the fused adjoint step: a multiple of 8 (XCD mapping)
+  const int nfb = (int)((blocks_for((long)h->n_lo + h->n_hi, kBlock) + 7) / 8 * 8);
   const int nslot = nblk + nfb;
   if (!rc && !h->obj.gacc && (e = hipMalloc(&h->obj.gacc, (size_t)nslot * 5 * sizeof(double))) != hipSuccess)
     fail("gradients", e);
@@ -1312,37 +1313,24 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
                                h->obj.s_node_d + lo, (const double *)nullptr, n);
         }
       } seed_clear{h, s_lo, s_n};
-      dim3 grid(nblk), block(kBlock);
+      // with the fused face conditions the first nfb blocks take the face
+      // nodes (their conditions formed from the state, transposed before the
+      // store); otherwise every node, and the transposes follow
+      const int fb = fz ? nfb : 0;
+      dim3 grid(nblk + fb), block(kBlock);
       Prof pr(h, 5);
       if (odd) {
-        if (off32) hipLaunchKernelGGL((k_adjoint_step<true, true>), grid, block, 0, h->stream, J);
-        else hipLaunchKernelGGL((k_adjoint_step<true, false>), grid, block, 0, h->stream, J);
+        if (off32) hipLaunchKernelGGL((k_adjoint_step<true, true>), grid, block, 0, h->stream, J, fb);
+        else hipLaunchKernelGGL((k_adjoint_step<true, false>), grid, block, 0, h->stream, J, fb);
       } else {
-        if (off32) hipLaunchKernelGGL((k_adjoint_step<false, true>), grid, block, 0, h->stream, J);
-        else hipLaunchKernelGGL((k_adjoint_step<false, false>), grid, block, 0, h->stream, J);
+        if (off32) hipLaunchKernelGGL((k_adjoint_step<false, true>), grid, block, 0, h->stream, J, fb);
+        else hipLaunchKernelGGL((k_adjoint_step<false, false>), grid, block, 0, h->stream, J, fb);
       }
       if ((e = hipGetLastError()) != hipSuccess) {
         fail("adjoint step", e);
         break;
       }
-      if (fz) {
-        // the face nodes: adjoint step with the face conditions and their transpose
-        if (nfb) {
-          dim3 fgrid(nfb);
-          if (odd) {
-            if (off32) hipLaunchKernelGGL((k_adjoint_face<true, true>), fgrid, block, 0, h->stream, J, nblk);
-            else hipLaunchKernelGGL((k_adjoint_face<true, false>), fgrid, block, 0, h->stream, J, nblk);
-          } else {
-            if (off32) hipLaunchKernelGGL((k_adjoint_face<false, true>), fgrid, block, 0, h->stream, J, nblk);
-            else hipLaunchKernelGGL((k_adjoint_face<false, false>), fgrid, block, 0, h->stream, J, nblk);
-          }
-          if ((e = hipGetLastError()) != hipSuccess) {
-            fail("adjoint face step", e);
-            break;
-          }
-        }
-        continue;
-      }
+      if (fz) continue;
       const lbm_params &p = h->p;
       if (p.zface_pressure || p.zface_solute) {
         for (int top = 0; top < 2; top++) {

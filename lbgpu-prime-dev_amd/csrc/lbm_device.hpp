@@ -124,10 +124,8 @@ struct StepArgs {
 // Workgroups are dispatched round-robin over the 8 XCDs; map them so that
 // each XCD sweeps one contiguous range of fluid nodes (its own L2 keeps the
 // partial lines that neighbouring waves share).
-__device__ __forceinline__ int xcd_block(int nblk8) {
-  int b = blockIdx.x;
-  return (b & 7) * nblk8 + (b >> 3);
-}
+__device__ __forceinline__ int xcd_block(int nblk8, int b) { return (b & 7) * nblk8 + (b >> 3); }
+__device__ __forceinline__ int xcd_block(int nblk8) { return xcd_block(nblk8, (int)blockIdx.x); }
 
 template <int Q, typename ST, typename CT, int SUBS>
 __device__ __forceinline__ CT shift_of(const StepArgs<ST, CT> &A, int a) {
