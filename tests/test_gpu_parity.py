@@ -377,3 +377,36 @@ def test_fp32_collision_permeability_within_1e_6():
     err = abs(k[1] - k[0]) / abs(k[0])
     say(f"permeability FP64 {k[0]:.10e} FP32 {k[1]:.10e} relative {err:.2e}")
     assert err <= 1e-6
+
+
+@pytest.mark.parametrize("compute", [0, 1])
+def test_2d_reference_bcs_fp32_storage(compute):
+    """D2Q9 with the reference's N/S pressure rows and zero-gradient solute
+    rows in FP32-shifted storage (FP64 or FP32 collision) against the FP64
+    oracle: the FP32 paths of the 2-D boundary kernels form the density
+    offset in FP64, so the mean flow stays within 1e-6"""
+    say(f"case 2D BCs FP32 storage compute={compute}")
+    L = _pkg.load()
+    ni, nj = 40, 64
+    mask, _ = L.discs2d(ni, nj, 31, 2.0, 4.0, 0.75, walls=False)
+    mask = mask.copy()
+    mask[:3] = 0
+    mask[-3:] = 0
+    kw = dict(dims=2, ni=ni, nj=nj, nsubs=2, tau=(0.8, 0.75), pressure_s_in=(1, 0), pressure_n_out=(1, 0),
+              rho_in=1.001, rho_out=1.0, zeroconcgrad_n=1, zeroconcgrad_s=1)
+    o = Oracle(**kw)
+    o.set_solids(mask)
+    g = L.Lattice(mask=mask, storage=1, compute=compute, **kw)
+    c0 = np.zeros(mask.size)
+    c0[: ni * 8] = 1.0
+    o.init_rest(c0)
+    g.init_rest(c0)
+    o.run(2000)
+    g.run(2000)
+    fl = mask.reshape(-1) != 1
+    mo, mg = o.mv(0), g.mv(0)
+    ey = relnorm(mg[2][fl], mo[2][fl])
+    em = abs(mg[2][fl].sum() / mo[2][fl].sum() - 1)
+    ec = relnorm(g.mv(1)[0][fl], o.mv(1)[0][fl])
+    say(f"compute={compute}: u_y {ey:.2e}, mean u_y {em:.2e}, c {ec:.2e}")
+    assert em <= 1e-6 and ey <= 1e-5 and ec <= 1e-5
