@@ -59,7 +59,7 @@ struct lbm_handle {
   int nf = 0;
   long tiles = 0;  // kTile-node tiles incl. ghost entries
   int W = 0;       // 64-node words per lattice row
-  int nblk8 = 0;
+  int nblk8 = 0;  // blocks of kBlock threads / 8
   size_t st_bytes = 8;
   void *F = nullptr;
   RowWord *rw = nullptr;
@@ -331,7 +331,9 @@ int launch_half(lbm_handle *h, int odd, int t, int write_mv, bool fuse_bc = fals
     A.solute_on = NS == 2 && t >= h->p.sigma_t_on && t <= h->p.sigma_t_off;
     Prof pr(h, odd ? 0 : 1);
     const bool off32 = !h->off64 && (size_t)h->tiles * NS * Q * kTile * sizeof(ST) <= 0xffffffffull;
-    dim3 grid(8 * h->nblk8), block(kBlock);
+    const int bs = odd ? half_step_block<true, ST>() : half_step_block<false, ST>();
+    A.nblk8 = (int)((blocks_for(h->nf, bs) + 7) / 8);
+    dim3 grid(8 * A.nblk8), block(bs);
     if (h->ns) {
       // Walsh partial bounce-back
       A.ns = h->ns;
@@ -1267,7 +1269,9 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
       if (fz) set_zb(h, A);
       A.Fo = (double *)seg[q];
       A.solute_on = 1;
-      dim3 grid(nblk), block(kBlock);
+      const int bs = (t & 1) ? half_step_block<true, double>() : half_step_block<false, double>();
+      A.nblk8 = (int)((blocks_for(h->nf, bs) + 7) / 8);
+      dim3 grid(8 * A.nblk8), block(bs);
       Prof pr(h, (t & 1) ? 0 : 1);
       if (t & 1) {
         if (off32) hipLaunchKernelGGL((k_half_step<19, 2, double, double, true, true>), grid, block, 0, h->stream, A);
