@@ -28,6 +28,7 @@ struct AdjArgs {
                                // unless A.zb.on: then the face nodes' boundary
                                // values are formed here from their known ones
   double *L;                   // adjoint state, same tile layout as the populations
+  const int *amap;             // node block of each launched off-face block, or null
   double seed;                 // dJ / d(plane flux of this step); 0 outside the window
   double kseed;                // dJ / d(z momentum of a node, this step); 0 outside the
                                // window or for the flux objective
@@ -387,8 +388,11 @@ __device__ __forceinline__ void adj_face_block(const AdjArgs &J, long slot, int 
 // Adjoint of one half-step.  With A.zb.on the first nfb blocks (a multiple
 // of 8, so the XCD mapping of the others is unchanged) take the z-face
 // nodes (adj_face_block) and the others every node off the faces; without
-// it (nfb = 0) every block takes the nodes of its range.  Gradient partials:
-// slot b of face block b, slot nfb + logical block of the others.
+// it (nfb = 0) every block takes the nodes of its range.  The node block of
+// the others comes from the row-band map J.amap (a permutation of all
+// 8 * nblk8 blocks) when there is one, else from the XCD-contiguous remap.
+// Gradient partials: slot b of face block b, slot nfb + node block of the
+// others (so the block order does not change the sums).
 template <bool ODD, bool OFF32>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ODD ? LBM_ADJ_WAVES_ODD : LBM_ADJ_WAVES_EVEN)))
 void k_adjoint_step(AdjArgs J, int nfb) {
@@ -396,7 +400,7 @@ void k_adjoint_step(AdjArgs J, int nfb) {
     adj_face_block<ODD, OFF32>(J, blockIdx.x, blockIdx.x);
     return;
   }
-  const int lb = xcd_block(J.A.nblk8, (int)blockIdx.x - nfb);
+  const int lb = J.amap ? J.amap[blockIdx.x - nfb] : xcd_block(J.A.nblk8, (int)blockIdx.x - nfb);
   const int idx = lb * kBlock + (int)threadIdx.x;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (idx < J.A.nf && !zface_node<19>(J.A, idx)) {

@@ -120,6 +120,7 @@ struct StepArgs {
   Geo G;
   int nf;         // fluid nodes
   int nblk8;      // launched blocks / 8
+  const int *bmap; // node block of each launched block (-1: none), or null
   double *mv;     // [nsubs][1+D][nf]
   int write_mv;
   int solute_on;
@@ -482,7 +483,11 @@ __device__ __forceinline__ void walsh_mix(CT (&f)[Q], const CT (&fp)[Q], CT ns, 
 
 template <int Q, int NS, typename ST, typename CT, bool ODD, bool OFF32, bool WALSH = false>
 __global__ __launch_bounds__((half_step_block<ODD, ST>())) void k_half_step(StepArgs<ST, CT> A) {
-  const int idx = xcd_block(A.nblk8) * half_step_block<ODD, ST>() + (int)threadIdx.x;
+  // node block: from the row-band map when the launch carries one, else the
+  // XCD-contiguous remap
+  const int lb = A.bmap ? A.bmap[blockIdx.x] : xcd_block(A.nblk8);
+  if (lb < 0) return;
+  const int idx = lb * half_step_block<ODD, ST>() + (int)threadIdx.x;
   if (idx >= A.nf) return;
   int nb[Q];
   nb[0] = idx;

@@ -184,6 +184,32 @@ def test_3d_zface_pressure_solute_flux(halo):
     assert relnorm(sg, so) <= 1e-12
 
 
+@pytest.mark.parametrize("nsubs", [1, 2])
+def test_row_band_block_order(nsubs):
+    """a wide, shallow lattice where one XCD's band of a plane holds at least
+    3 blocks of 512 nodes (nf / nk / 8 >= 1536), so the odd half-step runs in
+    the row-band block order: 24 steps with z-face conditions equal the
+    oracle bit for bit"""
+    say(f"case row-band block order nsubs={nsubs}")
+    L = _pkg.load()
+    ni, nj, nk = 256, 160, 10
+    mask, _ = L.spheres3d(ni, nj, nk, 606, 3.0, 6.0, 0.45, zbuf=2)
+    nf = int((mask.reshape(-1) != 1).sum())
+    assert nf / nk / 8 >= 3 * 512, nf
+    kw = dict(dims=3, ni=ni, nj=nj, nk=nk, nsubs=nsubs, tau=(0.8, 0.75), zface_pressure=1, rho_bot=1.001,
+              rho_top=1.0, gaccel=(1e-6, -2e-6, 0, 0, 0, 0))
+    if nsubs == 2:
+        kw.update(zface_solute=1, c_bot=1.0, flux_plane=nk - 2)
+    o, g = build_pair(mask, **kw)
+    f, gg = random_state(np.random.default_rng(66), 19, ni * nj * nk, nsubs)
+    load_both(o, g, f, gg)
+    o.run(24)
+    g.run(24)
+    _, bitwise = compare(o, g, mask, nsubs, f"row-band order nsubs={nsubs}")
+    assert bitwise
+    g.close()
+
+
 @pytest.mark.parametrize("dims", [2, 3])
 def test_pest_partial_macro_write_and_upload(dims):
     """LBM_MACRO_PEST writes the flow velocity and the solute concentration
