@@ -145,6 +145,48 @@ def permeability(sums, n_nodes, tau, rho_in, rho_out, length):
     return nu * (sums[3] / n_nodes) / ((rho_in - rho_out) / 3.0 / length)
 
 
+def throughput_c3(L, n=192, steps=1000):
+    """configuration 3 as a throughput workload: D3Q19 flow + solute 192^3,
+    porosity 0.45 (seed 303), z-face pressure and solute conditions, in FP64,
+    FP32-shifted (FP32 collision) and FP16-scaled storage"""
+    mask, phi = L.spheres3d(n, n, n, 303, 6.0, 14.0, 0.45, zbuf=4)
+    base = dict(dims=3, ni=n, nj=n, nk=n, nsubs=2, tau=(0.8, 0.75), zface_pressure=1, rho_bot=1.001,
+                rho_top=1.0, zface_solute=1, c_bot=1.0, flux_plane=n - 2)
+    out = {"lattice": [n, n, n], "porosity": round(phi, 5), "steps": steps}
+    for name, st, ct, s in (("fp64", 0, 0, 8), ("fp32_shifted", 1, 1, 4), ("fp16_scaled", 2, 1, 2)):
+        kw = dict(base, storage=st, compute=ct, store_shift_c=0.5 if st == 2 else 0.0)
+        info, wall, prof, _ = run_workload(L, mask, kw, steps, 10, f"c3 {name}")
+        B = algorithmic_bytes(info["fluid"], info["nodes"], 19, 2, s)
+        o_ms = prof["odd"][0] / max(prof["odd"][1], 1)
+        e_ms = prof["even"][0] / max(prof["even"][1], 1)
+        out[name] = {"mflups": round(info["fluid"] * steps / wall / 1e6, 1),
+                     "ms_per_step": round(wall / steps * 1e3, 4),
+                     "odd_frac": round(B / (o_ms * 1e-3) / HBM_PEAK, 4),
+                     "even_frac": round(B / (e_ms * 1e-3) / HBM_PEAK, 4),
+                     "step_frac_wall": round(B * steps / wall / HBM_PEAK, 4)}
+    out["fp32_over_fp64_speed"] = round(out["fp32_shifted"]["mflups"] / out["fp64"]["mflups"], 3)
+    return out
+
+
+def throughput_c1(L, n=128, steps=10000):
+    """configuration 1: D2Q9 FP64 128^2, discs to porosity 0.70 (seed 101)
+    between walls at y = 0 and 127, tau = 1, g_x = 1e-6, the full 10,000
+    steps from rest (the lattice fits in cache; reported as MFLUPS only)"""
+    mask, phi = L.discs2d(n, n, 101, 4.0, 10.0, 0.70, walls=True)
+    lat = L.Lattice(mask=mask, dims=2, ni=n, nj=n, tau=(1.0, 1.0), gaccel=(1e-6, 0, 0, 0, 0, 0))
+    info = lat.info()
+    lat.init_rest()
+    lat.run(100, write_last=False)
+    lat.init_rest()
+    t0 = time.perf_counter()
+    lat.run(steps, write_last=True)
+    wall = time.perf_counter() - t0
+    lat.close()
+    log(f"[c1] {steps} steps {wall:.3f}s")
+    return {"lattice": [n, n], "porosity": round(phi, 5), "fluid_nodes": info["fluid"], "steps": steps,
+            "mflups": round(info["fluid"] * steps / wall / 1e6, 1), "us_per_step": round(wall / steps * 1e6, 2)}
+
+
 def precision_table(L, n=192, steps=4000):
     """configuration 3: D3Q19 flow + solute 192^3, porosity 0.45 (seed 303),
     solute c = 1 at the inlet, zero gradient at the outlet, 4000 steps in
@@ -397,6 +439,8 @@ def main():
         result["extra"].update(ext)
         if not args.variants_only:
             result["extra"]["adjoint_c4"] = adjoint_timing(L)
+            result["extra"]["throughput_c3"] = throughput_c3(L)
+            result["extra"]["throughput_c1"] = throughput_c1(L)
             result["extra"]["precision_c3"] = precision_table(L)
             result["extra"]["slab_c5"] = slab_c5(L)
 
