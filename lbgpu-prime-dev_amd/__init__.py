@@ -91,6 +91,7 @@ def lib():
             "lbm_upload_macro": [vp, ip, dp, dp, dp, dp],
             "lbm_set_ns": [vp, dp],
             "lbm_objective_select": [vp, ip],
+            "lbm_objective_keep_states": [vp, ctypes.c_size_t],
             "lbm_sample_macro": [vp, ip, ip, ctypes.POINTER(ctypes.c_long), dp],
             "lbm_reduce": [vp, ip, ip, dp],
             "lbm_last_flux": [vp, dp, dp],
@@ -349,6 +350,11 @@ class Lattice:
         J = ctypes.c_double()
         _check(lib().lbm_objective_run(self.h, int(nsteps), int(window), int(every), ctypes.byref(J)))
         return J.value
+
+    def keep_states(self, max_bytes):
+        """device memory the next objective runs may spend on states kept
+        for the adjoint (lbm_objective_keep_states)"""
+        _check(lib().lbm_objective_keep_states(self.h, int(max_bytes)))
 
     def adjoint(self, with_map=True):
         """(dJ/d(gx, gy, gz, tau, tau_s), dJ/dc_bot_map or None)"""

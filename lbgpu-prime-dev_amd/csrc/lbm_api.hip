@@ -65,6 +65,7 @@ struct lbm_handle {
   int *amap = nullptr;  // row-band block map of the adjoint step (kBlock threads)
   size_t st_bytes = 8;
   void *F = nullptr;
+  void *Fout = nullptr;  // where the half-steps write when not F (out-of-place steps)
   RowWord *rw = nullptr;
   double *mv = nullptr;
   int *node = nullptr;  // compact -> dense
@@ -97,6 +98,9 @@ struct lbm_handle {
     int kind = LBM_OBJ_OUTLET_FLUX;  // the window term (lbm_objective_select)
     int t0 = 0, nsteps = 0, window = 0, every = 0;
     std::vector<void *> ckpt;  // state before steps t0+1, t0+every+1, ...
+    size_t keep_bytes = 0;     // lbm_objective_keep_states
+    std::vector<void *> kept;  // states (after the BCs) before the last kept steps of the run
+    int kept_from = 0;         // the run's first kept step index (0-based; nsteps: none)
     double *jacc = nullptr;
     double J = 0.0;
     double Jwin = 0.0;         // the window term of J
@@ -158,7 +162,7 @@ StepArgs<ST, CT> make_args(const lbm_handle *h) {
   StepArgs<ST, CT> A;
   const lbm_params &p = h->p;
   A.F = (ST *)h->F;
-  A.Fo = (ST *)h->F;
+  A.Fo = (ST *)(h->Fout ? h->Fout : h->F);
   A.G.rw = h->rw;
   A.G.node = h->node;
   A.G.ni = p.ni;
@@ -255,6 +259,9 @@ void free_objective(lbm_handle *h) {
   for (void *q : h->obj.seg)
     if (q) (void)hipFree(q);
   h->obj.seg.clear();
+  for (void *q : h->obj.kept)
+    if (q) (void)hipFree(q);
+  h->obj.kept.clear();
   void *bufs[] = {h->obj.jacc, h->obj.L, h->obj.gacc, h->obj.cgrad, h->obj.o_node_d, h->obj.o_c,
                   h->obj.s_node_d, h->obj.s_val_d, h->obj.seed};
   for (void *q : bufs)
@@ -1145,6 +1152,12 @@ extern "C" int lbm_objective_select(lbm_handle *h, int kind) {
   return 0;
 }
 
+extern "C" int lbm_objective_keep_states(lbm_handle *h, size_t max_bytes) {
+  if (!h) return lbm_fail("null handle");
+  h->obj.keep_bytes = max_bytes;
+  return 0;
+}
+
 extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int every, double *J) {
   if (need_lattice(h) || adjoint_supported(h)) return 1;
   const int nobs = (int)h->obj.o_step.size();
@@ -1160,7 +1173,33 @@ extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int ever
     return lbm_fail("the permeability needs a driving gradient (z-face pressure difference or g_z)");
   h->obj.valid = 0;
   const size_t sb = state_bytes(h);
-  const size_t nck = (size_t)((nsteps + every - 1) / every);
+  // the last P steps keep their states (P = nsteps - m * every, the fewest
+  // segments m left to replay that the budget allows); the m segments
+  // before them keep a checkpoint each
+  long P = 0;
+  if (h->obj.keep_bytes >= sb) {
+    long cap = (long)(h->obj.keep_bytes / sb);
+    // no more than leaves room for the adjoint's own buffers (every + 2
+    // states) and 1 GiB besides, counting the pool already held
+    size_t free_b = 0, total_b = 0;
+    HIP_OK(hipMemGetInfo(&free_b, &total_b));
+    const double room = (double)free_b + (double)h->obj.kept.size() * (double)sb -
+                        (double)(every + 2) * (double)sb - (double)(1ull << 30);
+    cap = std::min(cap, room > 0 ? (long)(room / (double)sb) : 0L);
+    P = cap >= nsteps ? nsteps : nsteps - (long)(nsteps - cap + every - 1) / every * every;
+    if (P < 0) P = 0;
+  }
+  const int kept_from = nsteps - (int)P;
+  while ((long)h->obj.kept.size() > P) {
+    (void)hipFree(h->obj.kept.back());
+    h->obj.kept.pop_back();
+  }
+  while ((long)h->obj.kept.size() < P) {
+    void *c = nullptr;
+    HIP_OK(hipMalloc(&c, sb));
+    h->obj.kept.push_back(c);
+  }
+  const size_t nck = (size_t)((kept_from + every - 1) / every);  // kept_from: nsteps or a multiple of every
   while (h->obj.ckpt.size() > nck) {
     (void)hipFree(h->obj.ckpt.back());
     h->obj.ckpt.pop_back();
@@ -1175,9 +1214,26 @@ extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int ever
   auto &o = h->obj;
   const double *mv_c = h->mv + (long)1 * (1 + h->D) * h->nf;  // solute concentration
   int ob = 0;
+  void *const live = h->F;
+  // the kept steps run out of place: kept[i] -> kept[i + 1], the last one
+  // into the lattice's own array
+  struct Restore {
+    lbm_handle *h;
+    void *F;
+    ~Restore() {
+      h->F = F;
+      h->Fout = nullptr;
+    }
+  } restore{h, live};
   for (int q = 0; q < nsteps; q++) {
-    if (q % every == 0)
+    if (q < kept_from && q % every == 0)
       HIP_OK(hipMemcpyAsync(o.ckpt[q / every], h->F, sb, hipMemcpyDeviceToDevice, h->stream));
+    if (q >= kept_from) {
+      const int i = q - kept_from;
+      if (i == 0) HIP_OK(hipMemcpyAsync(o.kept[0], live, sb, hipMemcpyDeviceToDevice, h->stream));
+      h->F = o.kept[i];
+      h->Fout = i + 1 < (int)P ? o.kept[i + 1] : live;
+    }
     const bool in_window = q >= nsteps - window;
     int oe = ob;
     while (oe < nobs && o.o_step[oe] == q + 1) oe++;
@@ -1201,6 +1257,8 @@ extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int ever
     }
     ob = oe;
   }
+  h->F = live;
+  h->Fout = nullptr;
   double acc = 0.0;
   HIP_OK(hipMemcpyAsync(&acc, o.jacc, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   std::vector<double> c(nobs);
@@ -1242,6 +1300,7 @@ extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int ever
     HIP_OK(hipMemcpy(o.s_val_d, sv.data(), sv.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   o.t0 = t0;
+  o.kept_from = kept_from;
   o.nsteps = nsteps;
   o.window = window;
   o.every = every;
@@ -1267,7 +1326,10 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
   if (!h->obj.valid) return lbm_fail("lbm_objective_run has not been called");
   const size_t sb = state_bytes(h);
   const int K = h->obj.every, T = h->obj.nsteps, W_ = h->obj.window, t0 = h->obj.t0;
-  const int nseg = (int)h->obj.ckpt.size();
+  const int nseg = (T + K - 1) / K;
+  // segments replayed from a checkpoint (kept_from is nsteps or a multiple
+  // of K); the others were kept by the objective run
+  const int nrep = (h->obj.kept_from + K - 1) / K;
   const int saved_time = h->time;
   void *const saved_F = h->F;  // the replay points h->F at segment buffers
   const int nblk = 8 * h->nblk8;
@@ -1280,11 +1342,12 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
   // workspace: K + 1 segment states, the adjoint state and the gradient
   // accumulators, kept for the next call
   std::vector<void *> &seg = h->obj.seg;
-  while (seg.size() > (size_t)K + 1) {
+  const size_t nsegbuf = nrep > 0 ? (size_t)K + 1 : 0;
+  while (seg.size() > nsegbuf) {
     (void)hipFree(seg.back());
     seg.pop_back();
   }
-  while (!rc && seg.size() < (size_t)K + 1) {
+  while (!rc && seg.size() < nsegbuf) {
     void *q = nullptr;
     if ((e = hipMalloc(&q, sb)) != hipSuccess) fail("segment buffers", e);
     else seg.push_back(q);
@@ -1319,12 +1382,16 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
   for (int sgi = nseg - 1; sgi >= 0 && !rc; sgi--) {
     const int ts = t0 + sgi * K;                 // state seg[0] is before step ts+1
     const int len = std::min(K, t0 + T - ts);
-    if ((e = hipMemcpyAsync(seg[0], h->obj.ckpt[sgi], sb, hipMemcpyDeviceToDevice, h->stream)) != hipSuccess) {
+    // the states before the segment's steps, after their BCs: kept by the
+    // objective run, or replayed from the segment's checkpoint
+    void *const *sp = sgi >= nrep ? h->obj.kept.data() + ((long)sgi * K - h->obj.kept_from) : seg.data();
+    if (sgi < nrep && (e = hipMemcpyAsync(seg[0], h->obj.ckpt[sgi], sb, hipMemcpyDeviceToDevice, h->stream)) !=
+                          hipSuccess) {
       fail("checkpoint restore", e);
       break;
     }
     // replay the segment out of place, keeping every state after its BCs
-    for (int q = 1; q <= len && !rc; q++) {
+    for (int q = 1; q <= len && !rc && sgi < nrep; q++) {
       const int t = ts + q;
       h->F = seg[q - 1];
       const bool fz = zface_fused(h, t);
@@ -1354,7 +1421,7 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
       const int t = ts + q;
       const bool odd = t & 1;
       AdjArgs J;
-      h->F = seg[q - 1];
+      h->F = sp[q - 1];
       const bool fz = zface_fused(h, t);
       J.A = make_args<double, double>(h);
       J.amap = h->amap;

@@ -220,3 +220,42 @@ def test_permeability_adjoint_matches_tlm_and_differences():
     _, Jf = gpu_objective(L, mask, cmap, PARAMS, 60, 20, 25)
     J_flux, _ = tlm_objective(mask, 1.001, 1.0, nk - 2, 60, 20, PARAMS, cmap)
     assert abs(Jf - J_flux) <= 1e-12 * abs(J_flux)
+
+
+@pytest.mark.parametrize("case", ["half_kept", "all_kept", "short_last_segment_kept", "none_kept"])
+def test_kept_states_give_the_same_gradient(case):
+    """lbm_objective_keep_states: the objective run keeps the states of its
+    last whole segments and the sweep uses them instead of replaying; J, the
+    gradients, the inlet-map gradient and the lattice's state after the run
+    equal those of the replaying sweep bit for bit"""
+    say(f"case kept states {case}")
+    L, mask, cmap = setup()
+    # (nsteps, window, every, states the budget holds): 40 of 80 steps kept;
+    # all 70 (the last of 3 segments has 20 steps); only the last segment of
+    # 10 steps (60 = 2 x 25 + 10); none (9 states hold no whole segment)
+    nsteps, window, every, nkeep = {"half_kept": (80, 20, 20, 40), "all_kept": (70, 15, 25, 1000),
+                                    "short_last_segment_kept": (60, 20, 25, 10),
+                                    "none_kept": (60, 20, 25, 9)}[case]
+    out = []
+    for keep in (False, True):
+        nk, nj, ni = mask.shape
+        lat = L.Lattice(mask=mask, dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(PARAMS[3], PARAMS[4]),
+                        gaccel=tuple(PARAMS[:3]) + (0.0, 0.0, 0.0), zface_pressure=1, rho_bot=1.001,
+                        rho_top=1.0, zface_solute=1, c_bot_map=cmap, flux_plane=nk - 2)
+        if keep:
+            # one state: the population array, 2048-node tiles x 2 components x 19 x 8 B
+            nf = lat.info()["fluid"]
+            state = (nf + 2047) // 2048 * 2048 * 2 * 19 * 8
+            lat.keep_states(nkeep * state)
+        lat.init_rest()
+        J = lat.objective(nsteps, window, every)
+        grad, gmap = lat.adjoint()
+        f_after = lat.download_f(0), lat.download_f(1)
+        out.append((J, grad, gmap, f_after))
+        lat.close()
+    (J0, g0, m0, f0), (J1, g1, m1, f1) = out
+    say(f"J {J0:.15e} / {J1:.15e}; grad {g0} / {g1}")
+    assert J0 == J1
+    assert np.array_equal(g0, g1)
+    assert np.array_equal(m0, m1)
+    assert np.array_equal(f0[0], f1[0]) and np.array_equal(f0[1], f1[1])
