@@ -21,6 +21,7 @@
 #ifndef LBM_MI355X_H
 #define LBM_MI355X_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus

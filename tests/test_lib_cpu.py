@@ -34,6 +34,24 @@ def test_library_exports_header_symbols():
     assert "lbm_io_read_deck" in syms and "lbm_deck_main" in syms
 
 
+def test_headers_compile_as_c(tmp_path):
+    """the C ABI headers are plain C (the reference's host code is C): a C99
+    translation unit that includes both and links against nothing compiles
+    with warnings as errors"""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "use_headers.c"
+    src.write_text('#include "lbm_mi355x.h"\n#include "lbm_io.h"\n'
+                   "int use(lbm_handle *h) { lbm_params p = {0}; (void)p; return lbm_destroy(h); }\n")
+    r = subprocess.run([cc, "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I",
+                        os.path.join(ROOT, "include"), "-c", str(src), "-o", str(tmp_path / "use_headers.o")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def test_discs2d_properties_and_hash():
     L = _pkg.load()
     m, phi = L.discs2d(128, 128, 101, 4.0, 10.0, 0.70, walls=True)
