@@ -66,6 +66,12 @@ struct lbm_handle {
   size_t st_bytes = 8;
   void *F = nullptr;
   void *Fout = nullptr;  // where the half-steps write when not F (out-of-place steps)
+  // lbm_run's captured chunks of steps, one per starting parity
+  unsigned gen = 0;  // bumped when the geometry or the Walsh field changes
+  hipGraphExec_t chunk[2] = {nullptr, nullptr};
+  unsigned chunk_gen[2] = {0, 0};
+  void *chunk_F[2] = {nullptr, nullptr};
+  bool chunk_off = false;  // a capture failed: steps are launched one by one
   RowWord *rw = nullptr;
   double *mv = nullptr;
   int *node = nullptr;  // compact -> dense
@@ -277,7 +283,16 @@ void free_objective(lbm_handle *h) {
   h->obj.valid = 0;
 }
 
+void drop_chunks(lbm_handle *h) {
+  for (int par = 0; par < 2; par++) {
+    if (h->chunk[par]) (void)hipGraphExecDestroy(h->chunk[par]);
+    h->chunk[par] = nullptr;
+  }
+  h->gen++;
+}
+
 void free_lattice(lbm_handle *h) {
+  drop_chunks(h);
   free_objective(h);
   void *ptrs[] = {h->F, h->rw, h->mv, h->node, h->face_lo, h->face_hi, h->face_flux, h->plane_lo,
                   h->plane_hi, h->tgt_lo, h->tgt_hi, h->halo_pos, h->halo_neg, h->ns, h->bmap, h->amap};
@@ -777,8 +792,9 @@ static int transfer_f(lbm_handle *h, int subs, double *f1d, int up) {
 
 extern "C" int lbm_set_ns(lbm_handle *h, const double *ns) {
   if (need_lattice(h)) return 1;
+  HIP_OK(hipStreamSynchronize(h->stream));
+  drop_chunks(h);
   if (h->ns) {
-    HIP_OK(hipStreamSynchronize(h->stream));
     (void)hipFree(h->ns);
     h->ns = nullptr;
   }
@@ -838,9 +854,84 @@ extern "C" int lbm_step(lbm_handle *h, int write_macro) {
   return sync(h);
 }
 
+namespace {
+
+constexpr int kChunk = 64;  // steps per captured chunk
+
+// Whether the steps t1..t2 launch the same kernels with the same arguments
+// as any other stretch of steps of the same parity: no profiling events, no
+// halo exchange, 2-D boundary densities without a pressure series or a ramp
+// still running, and the solute inside (or outside) its time window
+// throughout.
+bool chunkable(const lbm_handle *h, int t1, int t2) {
+  const lbm_params &p = h->p;
+  if (h->profile || p.halo_mode || h->chunk_off) return false;
+  if (h->D == 2) {
+    for (const auto &s : h->series)
+      if (!s.empty()) return false;
+    if (p.bc_ramp_start >= 0 && p.bc_ramp_stop >= 0 && t1 < p.bc_ramp_stop) return false;
+  }
+  if (p.nsubs == 2) {
+    const bool in1 = t1 >= p.sigma_t_on && t1 <= p.sigma_t_off;
+    const bool in2 = t2 >= p.sigma_t_on && t2 <= p.sigma_t_off;
+    if (in1 != in2 || (!in1 && !(t2 < p.sigma_t_on || t1 > p.sigma_t_off))) return false;
+  }
+  return true;
+}
+
+// The HIP graph of kChunk steps from the handle's current time and state
+// array, captured from the step launches on first use and kept while the
+// geometry, the Walsh field and the state array stay the same; null when
+// capture is not possible (the steps are then launched one by one).
+hipGraphExec_t chunk_for(lbm_handle *h) {
+  const int par = (h->time + 1) & 1;
+  if (h->chunk[par] && h->chunk_gen[par] == h->gen && h->chunk_F[par] == h->F) return h->chunk[par];
+  if (h->chunk[par]) {
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipGraphExecDestroy(h->chunk[par]);
+  }
+  h->chunk[par] = nullptr;
+  const int t_save = h->time;
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    h->chunk_off = true;
+    return nullptr;
+  }
+  int rc = 0;
+  for (int i = 0; i < kChunk && !rc; i++) rc = do_step(h, 0);
+  const hipError_t e = hipStreamEndCapture(h->stream, &g);
+  h->time = t_save;
+  hipGraphExec_t x = nullptr;
+  if (rc || e != hipSuccess || !g || hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    h->chunk_off = true;
+    return nullptr;
+  }
+  (void)hipGraphDestroy(g);
+  h->chunk[par] = x;
+  h->chunk_gen[par] = h->gen;
+  h->chunk_F[par] = h->F;
+  return x;
+}
+
+}  // namespace
+
 extern "C" int lbm_run(lbm_handle *h, int nsteps, int write_macro_last) {
   if (need_lattice(h)) return 1;
-  for (int q = 0; q < nsteps; q++)
+  int q = 0;
+  // the steps before the last in captured chunks where their launches repeat
+  if (nsteps - 1 >= kChunk && chunkable(h, h->time + 1, h->time + nsteps - 1)) {
+    while (nsteps - 1 - q >= kChunk) {
+      hipGraphExec_t g = chunk_for(h);
+      if (!g) break;
+      HIP_OK(hipGraphLaunch(g, h->stream));
+      h->time += kChunk;
+      q += kChunk;
+    }
+  }
+  for (; q < nsteps; q++)
     if (do_step(h, q == nsteps - 1 ? write_macro_last : 0)) return 1;
   return sync(h);
 }

@@ -436,3 +436,56 @@ def test_2d_reference_bcs_fp32_storage(compute):
     ec = relnorm(g.mv(1)[0][fl], o.mv(1)[0][fl])
     say(f"compute={compute}: u_y {ey:.2e}, mean u_y {em:.2e}, c {ec:.2e}")
     assert em <= 1e-6 and ey <= 1e-5 and ec <= 1e-5
+
+
+@pytest.mark.parametrize("case", ["2d_pressure_rows_solute", "3d_zface_solute_walsh", "2d_ramp_not_captured"])
+def test_captured_step_chunks_equal_stepwise_launches(case):
+    """lbm_run launches its steps in captured HIP graphs of 64 steps where
+    the launches repeat; three runs (the graphs replayed several times, with
+    eager steps, a parity change and synchronisations between them) equal
+    the same steps launched one by one, bit for bit, after every run"""
+    say(f"case captured chunks {case}")
+    L = _pkg.load()
+    rng = np.random.default_rng(515)
+    if case.startswith("2d"):
+        ni, nj = 40, 32
+        mask = (rng.random((nj, ni)) < 0.15).astype(np.uint8)
+        mask[:2] = 0
+        mask[-2:] = 0
+        kw = dict(dims=2, ni=ni, nj=nj, nsubs=2, tau=(0.9, 0.75), zeroconcgrad_n=1, zeroconcgrad_s=1,
+                  pressure_s_in=(1, 0), pressure_n_out=(1, 0), rho_in=1.002, rho_out=0.999)
+        if case == "2d_ramp_not_captured":
+            kw.update(bc_ramp_start=3, bc_ramp_stop=500)
+        Q, N, ns = 9, ni * nj, None
+    else:
+        ni, nj, nk = 16, 12, 20
+        mask, _ = L.spheres3d(ni, nj, nk, 616, 2.0, 3.5, 0.6, zbuf=3)
+        kw = dict(dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(0.8, 0.75), zface_pressure=1, rho_bot=1.001,
+                  rho_top=1.0, zface_solute=1, c_bot=1.0, flux_plane=nk - 2, gaccel=(1e-6, 0, 2e-6, 0, 0, 0))
+        Q, N = 19, ni * nj * nk
+        ns = np.where(rng.random(N) < 0.1, 0.3, 0.0)
+    f, gg = random_state(rng, Q, N, 2)
+    lats = []
+    for _ in range(2):
+        lat = L.Lattice(mask=mask, **kw)
+        if ns is not None:
+            lat.set_ns(ns)
+        lat.upload_f(0, f)
+        lat.upload_f(1, gg)
+        lats.append(lat)
+    a, b = lats
+    for n in (150, 1, 200, 1, 129):
+        if n == 1:
+            a.step()
+        else:
+            a.run(n, write_last=True)
+        for q in range(n):
+            b.step(n > 1 and q == n - 1)
+        for s in range(2):
+            assert np.array_equal(a.download_f(s), b.download_f(s)), (n, s)
+        if n > 1:
+            for s in range(2):
+                assert all(np.array_equal(x, y) for x, y in zip(a.mv(s), b.mv(s))), (n, s)
+    say(f"{case}: equal after 481 steps")
+    a.close()
+    b.close()
