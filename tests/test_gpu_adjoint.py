@@ -222,7 +222,8 @@ def test_permeability_adjoint_matches_tlm_and_differences():
     assert abs(Jf - J_flux) <= 1e-12 * abs(J_flux)
 
 
-@pytest.mark.parametrize("case", ["half_kept", "all_kept", "short_last_segment_kept", "none_kept"])
+@pytest.mark.parametrize("case", ["half_kept", "all_kept", "short_last_segment_kept", "none_kept",
+                                  "half_kept_permeability_observations"])
 def test_kept_states_give_the_same_gradient(case):
     """lbm_objective_keep_states: the objective run keeps the states of its
     last whole segments and the sweep uses them instead of replaying; J, the
@@ -235,7 +236,11 @@ def test_kept_states_give_the_same_gradient(case):
     # 10 steps (60 = 2 x 25 + 10); none (9 states hold no whole segment)
     nsteps, window, every, nkeep = {"half_kept": (80, 20, 20, 40), "all_kept": (70, 15, 25, 1000),
                                     "short_last_segment_kept": (60, 20, 25, 10),
-                                    "none_kept": (60, 20, 25, 9)}[case]
+                                    "none_kept": (60, 20, 25, 9),
+                                    "half_kept_permeability_observations": (60, 20, 20, 30)}[case]
+    # the permeability window writes the velocity on its steps, the observed
+    # steps the concentration (in kept and in replayed segments)
+    kind = "permeability" if "permeability" in case else "flux"
     out = []
     for keep in (False, True):
         nk, nj, ni = mask.shape
@@ -247,8 +252,10 @@ def test_kept_states_give_the_same_gradient(case):
             nf = lat.info()["fluid"]
             state = (nf + 2047) // 2048 * 2048 * 2 * 19 * 8
             lat.keep_states(nkeep * state)
+        if "observations" in case:
+            lat.observations(*observations(mask))
         lat.init_rest()
-        J = lat.objective(nsteps, window, every)
+        J = lat.objective(nsteps, window, every, kind=kind)
         grad, gmap = lat.adjoint()
         f_after = lat.download_f(0), lat.download_f(1)
         out.append((J, grad, gmap, f_after))
