@@ -25,12 +25,14 @@
 namespace lbm {
 
 constexpr int kBlock = 256;
-// Workgroup size of the odd (pull/push) half-step with FP64 or FP32 storage.
-// Its 19 gathers per node keep more requests in flight per CU with 8-wave
-// workgroups: at 256^3 (D3Q19, FP64, porosity 0.40) the odd step takes
-// 0.437 ms at 512 threads against 0.457 ms at 256 and 0.491 ms at 1024.  The
-// even step, and the odd step with FP16 storage (0.269 ms at 256 threads,
-// 0.280 ms at 512, flow + solute), stay at 256.
+// Workgroup size of the odd (pull/push) half-step.  Its 19 gathers per node
+// keep more requests in flight per CU with 8-wave workgroups: at 256^3
+// (D3Q19, FP64, porosity 0.40) the odd step takes 0.437 ms at 512 threads
+// against 0.457 ms at 256 and 0.491 ms at 1024 (contiguous block order).
+// With FP16 storage (flow + solute) it stays at 256 threads in the
+// contiguous order: 512 threads cost 4 % there (0.269 -> 0.280 ms), and in
+// the row-band order the step pair gained 4 % at 256^3 but lost 4 % at
+// 192^3 (porosity 0.45).  The even step is best at 256.
 constexpr int kBlockOdd = 512;
 template <bool ODD, typename ST>
 constexpr int half_step_block() { return ODD && sizeof(ST) >= 4 ? kBlockOdd : kBlock; }
