@@ -349,7 +349,7 @@ def main():
                     help="of the extras, run only the 128^3 discrete-adjoint workload")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
     L = load_pkg()
     copy_gbps = L.copy_bandwidth(2 << 30, 20)

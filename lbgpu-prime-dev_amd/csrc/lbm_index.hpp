@@ -1,5 +1,5 @@
 // Index arithmetic shared by the host set-up, the device kernels and the
-// CPU index checker (tests/index_check.cpp compiles this header with g++).
+// CPU index test (tests/index_check.cpp compiles this header with g++).
 //
 // Populations live in tiles of kTile consecutive fluid nodes:
 //   one distribution:  element (slot a, node x) at

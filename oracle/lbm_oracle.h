@@ -1,9 +1,9 @@
 /*
  * lbm_oracle.h -- CPU oracle for the lbgpu-prime stream-collide time step.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP product
- * path is compared against.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load it; the product library never links,
+ * TEST INFRASTRUCTURE ONLY.  The HIP product path is compared against this
+ * library.  Only tests/, __graft_entry__.smoke() and the CPU baseline timing
+ * of bench.py may load it; the product library never links,
  * loads or falls back to it.
  *
  * It is a plain-C restatement, in FP64 with -ffp-contract=off, of the GPU

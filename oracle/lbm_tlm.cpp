@@ -1,5 +1,5 @@
 // lbm_tlm.cpp -- FP64 tangent-linear model of the D3Q19 flow + solute
-// objective.  TEST INFRASTRUCTURE (the checker of the GPU discrete adjoint);
+// objective.  TEST INFRASTRUCTURE (the reference the GPU discrete adjoint is compared with);
 // never shipped or measured as the product.
 //
 // The model is written once, generic in its number type, on the natural

@@ -1,7 +1,8 @@
 """ctypes binding of the CPU oracle (oracle/liblbm_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker, and by bench.py's cpu_baseline leg.  The product library never
+the reference the GPU result is compared with, and by bench.py's CPU baseline
+timing.  The product library never
 imports this module.
 """
 import ctypes
