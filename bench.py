@@ -14,6 +14,7 @@ FP16-scaled storage (FP32 collision), a device copy bandwidth, and the CPU
 oracle on a bounded sample of the default workload (cpu_baseline).
 """
 import argparse
+import hashlib
 import importlib.util
 import json
 import os
@@ -131,7 +132,8 @@ def adjoint_timing(L, n=128, steps=2000, window=200, every=100, keep_gb=200.0):
         f"(replay {replay_ms:.1f} ms, adjoint kernels {adj_ms:.1f} ms, bc {prof['bc'][0]:.1f} ms) grad {grad}")
     log(f"[adjoint] kept states ({keep_gb:.0f} GB): forward {t_fwd_k:.3f}s adjoint {t_adj_k:.3f}s, "
         f"same gradient {same}")
-    return {"lattice": [n, n, n], "porosity": round(phi, 5), "steps": steps, "window": window,
+    return {"lattice": [n, n, n], "porosity": round(phi, 5), "geometry_sha256": sha(mask), "steps": steps,
+            "window": window,
             "kept_states": {"budget_gb": keep_gb, "forward_s": round(t_fwd_k, 4), "adjoint_s": round(t_adj_k, 4),
                             "adjoint_over_forward": round(t_adj_k / t_fwd_k, 3),
                             "gradient_over_forward": round((t_fwd_k + t_adj_k) / t_fwd_k, 3),
@@ -166,14 +168,19 @@ def permeability(sums, n_nodes, tau, rho_in, rho_out, length):
     return nu * (sums[3] / n_nodes) / ((rho_in - rho_out) / 3.0 / length)
 
 
-def throughput_c3(L, n=192, steps=1000):
+def sha(mask):
+    return hashlib.sha256(np.ascontiguousarray(mask).tobytes()).hexdigest()
+
+
+def throughput_c3(L, n=192, steps=1000, cpu_s=0.0):
     """configuration 3 as a throughput workload: D3Q19 flow + solute 192^3,
     porosity 0.45 (seed 303), z-face pressure and solute conditions, in FP64,
-    FP32-shifted (FP32 collision) and FP16-scaled storage"""
+    FP32-shifted (FP32 collision) and FP16-scaled storage; with cpu_s > 0 the
+    CPU oracle (FP64) timed beside it on the same lattice for about cpu_s"""
     mask, phi = L.spheres3d(n, n, n, 303, 6.0, 14.0, 0.45, zbuf=4)
     base = dict(dims=3, ni=n, nj=n, nk=n, nsubs=2, tau=(0.8, 0.75), zface_pressure=1, rho_bot=1.001,
                 rho_top=1.0, zface_solute=1, c_bot=1.0, flux_plane=n - 2)
-    out = {"lattice": [n, n, n], "porosity": round(phi, 5), "steps": steps}
+    out = {"lattice": [n, n, n], "porosity": round(phi, 5), "geometry_sha256": sha(mask), "steps": steps}
     for name, st, ct, s in (("fp64", 0, 0, 8), ("fp32_shifted", 1, 1, 4), ("fp16_scaled", 2, 1, 2)):
         kw = dict(base, storage=st, compute=ct, store_shift_c=0.5 if st == 2 else 0.0)
         info, wall, prof, _ = run_workload(L, mask, kw, steps, 10, f"c3 {name}")
@@ -186,15 +193,23 @@ def throughput_c3(L, n=192, steps=1000):
                      "even_frac": round(B / (e_ms * 1e-3) / HBM_PEAK, 4),
                      "step_frac_wall": round(B * steps / wall / HBM_PEAK, 4)}
     out["fp32_over_fp64_speed"] = round(out["fp32_shifted"]["mflups"] / out["fp64"]["mflups"], 3)
+    if cpu_s > 0:
+        nf = int((mask.reshape(-1) != 1).sum())
+        samples, threads = cpu_baseline(mask, base, cpu_s)
+        rates = sorted(nf * k / el / 1e6 for k, el in samples)
+        out["cpu_oracle_fp64"] = {"mflups": round(rates[len(rates) // 2], 2), "threads": threads,
+                                  "samples": [f"{k} steps in {el:.1f} s" for k, el in samples]}
     return out
 
 
-def throughput_c1(L, n=128, steps=10000):
+def throughput_c1(L, n=128, steps=10000, cpu=False):
     """configuration 1: D2Q9 FP64 128^2, discs to porosity 0.70 (seed 101)
     between walls at y = 0 and 127, tau = 1, g_x = 1e-6, the full 10,000
-    steps from rest (the lattice fits in cache; reported as MFLUPS only)"""
+    steps from rest (the lattice fits in cache; reported as MFLUPS only);
+    with cpu the CPU oracle runs the same 10,000 steps"""
     mask, phi = L.discs2d(n, n, 101, 4.0, 10.0, 0.70, walls=True)
-    lat = L.Lattice(mask=mask, dims=2, ni=n, nj=n, tau=(1.0, 1.0), gaccel=(1e-6, 0, 0, 0, 0, 0))
+    kw = dict(dims=2, ni=n, nj=n, tau=(1.0, 1.0), gaccel=(1e-6, 0, 0, 0, 0, 0))
+    lat = L.Lattice(mask=mask, **kw)
     info = lat.info()
     lat.init_rest()
     lat.run(100, write_last=False)
@@ -204,8 +219,21 @@ def throughput_c1(L, n=128, steps=10000):
     wall = time.perf_counter() - t0
     lat.close()
     log(f"[c1] {steps} steps {wall:.3f}s")
-    return {"lattice": [n, n], "porosity": round(phi, 5), "fluid_nodes": info["fluid"], "steps": steps,
-            "mflups": round(info["fluid"] * steps / wall / 1e6, 1), "us_per_step": round(wall / steps * 1e6, 2)}
+    out = {"lattice": [n, n], "porosity": round(phi, 5), "geometry_sha256": sha(mask), "fluid_nodes": info["fluid"],
+           "steps": steps, "mflups": round(info["fluid"] * steps / wall / 1e6, 1),
+           "us_per_step": round(wall / steps * 1e6, 2)}
+    if cpu:
+        sys.path.insert(0, ROOT)
+        from oracle import pyoracle
+        o = pyoracle.Oracle(**kw)
+        o.set_solids(mask)
+        o.init_rest()
+        t0 = time.perf_counter()
+        o.run(steps, write_last=True)
+        el = time.perf_counter() - t0
+        out["cpu_oracle_fp64"] = {"mflups": round(info["fluid"] * steps / el / 1e6, 2),
+                                  "threads": pyoracle.num_threads(), "seconds": round(el, 3)}
+    return out
 
 
 def precision_table(L, n=192, steps=4000):
@@ -295,6 +323,7 @@ def slab_c5(L, n=448, steps=2000):
                                  "permeability": abs(res["fp32_shifted"]["permeability"] - res["fp64"]["permeability"])
                                  / abs(res["fp64"]["permeability"])}
     res["lattice"] = [n, n, n]
+    res["geometry_sha256"] = sha(mask)
     res["porosity"] = round(phi, 5)
     res["steps"] = steps
     return res
@@ -390,7 +419,7 @@ def main():
         "config": {
             "workload": f"d3q19_flow_fp64_{n}cubed_spheres_phi0.40_zouhe",
             "lattice": [n, n, n], "porosity": round(phi, 5), "fluid_nodes": Nf, "tau": 0.8,
-            "rho_in": 1.001, "rho_out": 1.0, "parallelism": "single-gpu",
+            "rho_in": 1.001, "rho_out": 1.0, "parallelism": "single-gpu", "geometry_sha256": sha(mask),
         },
         "roofline": {
             "bound": "hbm", "kernel": f"k_half_step<19,1,double,double,{dom == 'odd'}>".replace("True", "true")
@@ -460,8 +489,8 @@ def main():
         result["extra"].update(ext)
         if not args.variants_only:
             result["extra"]["adjoint_c4"] = adjoint_timing(L)
-            result["extra"]["throughput_c3"] = throughput_c3(L)
-            result["extra"]["throughput_c1"] = throughput_c1(L)
+            result["extra"]["throughput_c3"] = throughput_c3(L, cpu_s=0.0 if args.no_cpu else 9.0)
+            result["extra"]["throughput_c1"] = throughput_c1(L, cpu=not args.no_cpu)
             result["extra"]["precision_c3"] = precision_table(L)
             result["extra"]["slab_c5"] = slab_c5(L)
 

@@ -79,6 +79,25 @@ def test_spheres3d_properties():
     assert not np.array_equal(m, m3)
 
 
+@pytest.mark.parametrize("name,args,sha", [
+    ("256^3 flow, porosity 0.40", (256, 256, 256, 202, 6.0, 14.0, 0.40),
+     "58d3c9c5de2e132090d9702f02a17a6e2bba6167678a723a13e2f3f19a259d0e"),
+    ("192^3 flow + solute, porosity 0.45", (192, 192, 192, 303, 6.0, 14.0, 0.45),
+     "a3f40ce3a7fcd4651967e742b88ae9fccfdd7d4d57f027e4f2731882d10a4e84"),
+    ("128^3 adjoint, porosity 0.40", (128, 128, 128, 404, 6.0, 14.0, 0.40),
+     "3c1fc171307dd4657989c624b4d869f5dac9a6e1640a4a983cee306d428ce890"),
+    ("448^3 slab, porosity 0.35", (448, 448, 448, 505, 8.0, 20.0, 0.35),
+     "a1c6f0410c42e6daad980d701bbb5adf040cbcc44965e8422acd9049b093b166"),
+])
+def test_benchmark_geometries_hash(name, args, sha):
+    """the seeded sphere packs of the benchmark configurations (4-plane fluid
+    buffers at each z face) are reproducible byte for byte"""
+    L = _pkg.load()
+    m, phi = L.spheres3d(*args, zbuf=4)
+    assert phi <= args[-1]
+    assert hashlib.sha256(m.tobytes()).hexdigest() == sha, name
+
+
 def test_geometry_rejects_bad_arguments():
     L = _pkg.load()
     with pytest.raises(L.LbmError):
