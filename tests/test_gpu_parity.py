@@ -438,7 +438,8 @@ def test_2d_reference_bcs_fp32_storage(compute):
     assert em <= 1e-6 and ey <= 1e-5 and ec <= 1e-5
 
 
-@pytest.mark.parametrize("case", ["2d_pressure_rows_solute", "3d_zface_solute_walsh", "2d_ramp_not_captured"])
+@pytest.mark.parametrize("case", ["2d_pressure_rows_solute", "3d_zface_solute_walsh", "2d_ramp_not_captured",
+                                  "3d_zface_solute_fp32", "3d_zface_solute_fp16"])
 def test_captured_step_chunks_equal_stepwise_launches(case):
     """lbm_run launches its steps in captured HIP graphs of 64 steps where
     the launches repeat; three runs (the graphs replayed several times, with
@@ -464,6 +465,12 @@ def test_captured_step_chunks_equal_stepwise_launches(case):
                   rho_top=1.0, zface_solute=1, c_bot=1.0, flux_plane=nk - 2, gaccel=(1e-6, 0, 2e-6, 0, 0, 0))
         Q, N = 19, ni * nj * nk
         ns = np.where(rng.random(N) < 0.1, 0.3, 0.0)
+        if case.endswith("fp32"):
+            kw.update(storage=1, compute=1)
+            ns = None
+        elif case.endswith("fp16"):
+            kw.update(storage=2, compute=1, store_shift_c=0.5)
+            ns = None
     f, gg = random_state(rng, Q, N, 2)
     lats = []
     for _ in range(2):
