@@ -57,8 +57,9 @@ __device__ __forceinline__ uint32_t push_off(const int (&nb)[19], int s, int a, 
 // waves (about 236 registers): squeezed to three it fits in 150 registers
 // only by reusing one register quadruple for all 19 stores, and each store
 // then waits for every memory access before it (233 us against 133 us per
-// launch at 128^3).  The odd step, with its neighbour ranks, ran faster at
-// one wave (about 280 registers) than squeezed to two with spills.
+// launch at 128^3).  The odd step, with its neighbour ranks, runs at one
+// wave (344 registers): squeezed to two (256 registers and 356 B of spills
+// per lane) the adjoint kernels of a 128^3 sweep took 313 ms instead of 303.
 #ifndef LBM_ADJ_WAVES_ODD
 #define LBM_ADJ_WAVES_ODD 1
 #endif
