@@ -1,17 +1,21 @@
 """Benchmark of the MI355X stream-collide step (one JSON line on stdout).
 
-Default workload (value): D3Q19 BGK flow, FP64, 256^3, tau = 0.8, seeded
-overlapping spheres (radius U[6,14], seed 202) to porosity 0.40 between
-4-plane solid-free buffers, Zou-He rho = 1.001 at z = 0 and 1.000 at
-z = 255, periodic in x and y, started from rest.  A step is one LBM time
-step (alternately the odd stream-collide-stream and the even collide
-half-step, each with its boundary kernels) over the whole lattice, with all
-populations resident in HBM.
+Default workload (value): D3Q19 BGK flow plus the D3Q19 solute distribution,
+FP64, 256^3, tau = 0.8, tau_s = 0.75, seeded overlapping spheres (radius
+U[6,14], seed 202) to porosity 0.40 between 4-plane solid-free buffers,
+Zou-He rho = 1.001 at z = 0 and 1.000 at z = 255, solute c = 1 at z = 0 and
+zero gradient at z = 255, periodic in x and y, started from rest.  A step is
+one LBM time step (alternately the odd stream-collide-stream and the even
+collide half-step, each with its boundary conditions) over the whole
+lattice, with all populations resident in HBM.
 
-Also measured in the same run and reported under "extra": D3Q19 flow +
-solute at 256^3 in FP64, FP32-shifted (FP32 or FP64 collision) and
-FP16-scaled storage (FP32 collision), a device copy bandwidth, and the CPU
-oracle on a bounded sample of the default workload (cpu_baseline).
+Also measured in the same run and reported under "extra": the flow-only
+lattice of the same geometry (FP64, its permeability after 5,000 steps),
+flow + solute at 256^3 in FP32-shifted (FP32 or FP64 collision) and
+FP16-scaled storage, the Walsh variant, the discrete adjoint at 128^3, the
+192^3 flow + solute and 128^2 D2Q9 configurations, the precision table, the
+448^3 slab, a device copy bandwidth, and the CPU oracle on a bounded sample
+of the default workload (cpu_baseline).
 """
 import argparse
 import hashlib
@@ -51,7 +55,10 @@ def algorithmic_bytes(n_fluid, n_nodes, q, nsubs, s):
     return nsubs * n_fluid * q * 2 * s + n_nodes
 
 
-def run_workload(L, mask, kw, steps, warmup, label, ns=None):
+def run_workload(L, mask, kw, steps, warmup, label, ns=None, total_steps=0):
+    """warmup steps, then `steps` timed steps (kernel times from HIP events
+    on the library's stream); with total_steps above warmup + steps the run
+    continues, untimed, to total_steps from rest before the sums are taken"""
     lat = L.Lattice(mask=mask, **kw)
     if ns is not None:
         lat.set_ns(ns)
@@ -67,6 +74,9 @@ def run_workload(L, mask, kw, steps, warmup, label, ns=None):
     wall = time.perf_counter() - t0
     prof = lat.profile_read()
     lat.profile(False)
+    if total_steps > warmup + steps:
+        lat.run(total_steps - warmup - steps, write_last=True)
+    info["steps_from_rest"] = max(total_steps, warmup + steps)
     sums = lat.reduce_sums()
     lat.close()
     log(f"[{label}] {steps} steps {wall:.3f}s kernels {prof}")
@@ -128,8 +138,9 @@ def adjoint_timing(L, n=128, steps=2000, window=200, every=100, keep_gb=250.0):
         t_adj_k = time.perf_counter() - t0
         same = bool(J_k == J and np.array_equal(grad_k, grad) and np.array_equal(gmap_k, gmap))
         kept.update({"forward_s": round(t_fwd_k, 4), "adjoint_s": round(t_adj_k, 4),
-                     "adjoint_over_forward": round(t_adj_k / t_fwd_k, 3),
-                     "gradient_over_forward": round((t_fwd_k + t_adj_k) / t_fwd_k, 3), "same_gradient_bits": same})
+                     "adjoint_over_plain_forward": round(t_adj_k / t_plain, 3),
+                     "gradient_over_plain_forward": round((t_fwd_k + t_adj_k) / t_plain, 3),
+                     "same_gradient_bits": same})
         log(f"[adjoint] kept states ({keep_gb:.0f} GB): forward {t_fwd_k:.3f}s adjoint {t_adj_k:.3f}s, "
             f"same gradient {same}")
     except L.LbmError as err:
@@ -137,6 +148,19 @@ def adjoint_timing(L, n=128, steps=2000, window=200, every=100, keep_gb=250.0):
         kept["error"] = str(err)
         log(f"[adjoint] kept states ({keep_gb:.0f} GB) failed: {err}")
     lat.close()
+    # the same objective at a plane the solute front reaches within the run
+    # (z = 8; at the outlet plane J is at round-off after 2000 steps)
+    lat = L.Lattice(mask=mask, dims=3, ni=n, nj=n, nk=n, nsubs=2, tau=(0.8, 0.75),
+                    gaccel=(0.0, 0.0, 1e-6, 0, 0, 0), zface_pressure=1, rho_bot=1.001, rho_top=1.0,
+                    zface_solute=1, c_bot_map=cmap, flux_plane=8)
+    lat.init_rest()
+    lat.time = 0
+    J8 = lat.objective(steps, window, every)
+    grad8, gmap8 = lat.adjoint()
+    lat.close()
+    reach = {"flux_plane": 8, "J": J8, "dJ_dgz": grad8[2], "dJ_dtau": grad8[3], "dJ_dtau_s": grad8[4],
+             "dJ_dcmap_sum": float(gmap8.sum())}
+    log(f"[adjoint] plane 8: J {J8:.6e} grad {grad8}")
     replay_ms = prof["odd"][0] + prof["even"][0]
     adj_ms = prof["adjoint"][0]
     log(f"[adjoint] J {J:.6e} forward {t_fwd:.3f}s (plain run {t_plain:.3f}s) adjoint {t_adj:.3f}s "
@@ -144,19 +168,22 @@ def adjoint_timing(L, n=128, steps=2000, window=200, every=100, keep_gb=250.0):
     return {"lattice": [n, n, n], "porosity": round(phi, 5), "geometry_sha256": sha(mask), "steps": steps,
             "window": window,
             "kept_states": kept,
-            "checkpoint_every": every, "J": J, "dJ_dgz": grad[2], "dJ_dtau": grad[3], "dJ_dtau_s": grad[4],
-            "dJ_dcmap_sum": float(gmap.sum()), "forward_s": round(t_fwd, 4), "plain_forward_s": round(t_plain, 4),
+            "checkpoint_every": every,
+            "outlet_plane_at_roundoff": {"flux_plane": n - 2, "J": J, "dJ_dgz": grad[2], "dJ_dtau": grad[3],
+                                         "dJ_dtau_s": grad[4], "dJ_dcmap_sum": float(gmap.sum())},
+            "reachable_plane": reach,
+            "forward_s": round(t_fwd, 4), "plain_forward_s": round(t_plain, 4),
             "adjoint_s": round(t_adj, 4), "adjoint_replay_ms": round(replay_ms, 2),
             "adjoint_kernels_ms": round(adj_ms, 2),
-            "adjoint_over_forward": round(t_adj / t_fwd, 3),
-            "gradient_over_forward": round((t_fwd + t_adj) / t_fwd, 3)}
+            "adjoint_over_plain_forward": round(t_adj / t_plain, 3),
+            "gradient_over_plain_forward": round((t_fwd + t_adj) / t_plain, 3)}
 
 
-def recorded_traffic(kernel_sig):
+def recorded_traffic(kernel_sig, name="hbm_traffic_d3q19_flow_solute_fp64_256.md"):
     """HBM bytes per launch of a kernel as recorded by the two --pmc runs
-    summarised in profiles/hbm_traffic_d3q19_flow_fp64_256.md (FETCH_SIZE and
-    WRITE_SIZE scaled by the same counters on a plain copy kernel)"""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_d3q19_flow_fp64_256.md")
+    summarised in profiles/<name> (FETCH_SIZE and WRITE_SIZE scaled by the
+    same counters on a plain copy kernel)"""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
             if kernel_sig in line and line.startswith("|"):
@@ -188,7 +215,7 @@ def throughput_c3(L, n=192, steps=1000, cpu_s=0.0):
                 rho_top=1.0, zface_solute=1, c_bot=1.0, flux_plane=n - 2)
     out = {"lattice": [n, n, n], "porosity": round(phi, 5), "geometry_sha256": sha(mask), "steps": steps}
     for name, st, ct, s in (("fp64", 0, 0, 8), ("fp32_shifted", 1, 1, 4), ("fp16_scaled", 2, 1, 2)):
-        kw = dict(base, storage=st, compute=ct, store_shift_c=0.5 if st == 2 else 0.0)
+        kw = dict(base, storage=st, compute=ct)
         info, wall, prof, _ = run_workload(L, mask, kw, steps, 10, f"c3 {name}")
         B = algorithmic_bytes(info["fluid"], info["nodes"], 19, 2, s)
         o_ms = prof["odd"][0] / max(prof["odd"][1], 1)
@@ -242,19 +269,23 @@ def throughput_c1(L, n=128, steps=10000, cpu=False):
     return out
 
 
-def precision_table(L, n=192, steps=4000):
+def precision_table(L, n=192, steps=4000, plane=16):
     """configuration 3: D3Q19 flow + solute 192^3, porosity 0.45 (seed 303),
     solute c = 1 at the inlet, zero gradient at the outlet, 4000 steps in
     each storage precision; errors against the FP64 run, which the tests
-    show equal to the FP64 CPU oracle bit for bit"""
+    show equal to the FP64 CPU oracle bit for bit.  The solute flux is taken
+    at plane z = `plane`, which the front passes within the run; the
+    specified outlet plane (z = n - 2) is reported from the FP64 run, where
+    after 4000 steps the flux is still at round-off (the front is 40-60
+    planes from the inlet), so its relative errors would mean nothing"""
     mask, phi = L.spheres3d(n, n, n, 303, 6.0, 14.0, 0.45, zbuf=4)
     base = dict(dims=3, ni=n, nj=n, nk=n, nsubs=2, tau=(0.8, 0.75), zface_pressure=1, rho_bot=1.001,
-                rho_top=1.0, zface_solute=1, c_bot=1.0, flux_plane=n - 2)
+                rho_top=1.0, zface_solute=1, c_bot=1.0)
     fl = mask.reshape(-1) != 1
     runs = {}
     for name, st, ct in (("fp64", 0, 0), ("fp64_store_fp32_collision", 0, 1), ("fp32_shifted", 1, 1),
                          ("fp32_shifted_fp64_collision", 1, 0), ("fp16_scaled", 2, 1)):
-        lat = L.Lattice(mask=mask, storage=st, compute=ct, store_shift_c=0.5 if st == 2 else 0.0, **base)
+        lat = L.Lattice(mask=mask, storage=st, compute=ct, flux_plane=plane, **base)
         lat.init_rest()
         t0 = time.perf_counter()
         lat.run(steps, write_last=True)
@@ -267,11 +298,17 @@ def precision_table(L, n=192, steps=4000):
         # the FP64 GPU reductions: flow mass sum(rho - rho_A), solute mass sum(c)
         runs[name] = (rho[fl] - 1.0, np.stack([ux[fl], uy[fl], uz[fl]]), c[fl], flux, k, wall, sums[0], sums[5])
         lat.close()
-        log(f"[precision] {name} {wall:.2f}s flux {flux:.6e} k {k:.6e}")
+        log(f"[precision] {name} {wall:.2f}s flux(z={plane}) {flux:.6e} k {k:.6e}")
+    lat = L.Lattice(mask=mask, flux_plane=n - 2, **base)
+    lat.init_rest()
+    lat.run(steps, write_last=True)
+    outlet = lat.last_flux()[0]
+    lat.close()
     ref = runs["fp64"]
     out = {"lattice": [n, n, n], "porosity": round(phi, 5), "steps": steps,
-           "permeability_fp64": ref[4], "outlet_flux_fp64": ref[3], "flow_mass_fp64": ref[6],
-           "solute_mass_fp64": ref[7]}
+           "permeability_fp64": ref[4], "flux_plane": plane, "flux_fp64": ref[3], "flow_mass_fp64": ref[6],
+           "solute_mass_fp64": ref[7],
+           "outlet_flux_fp64_at_roundoff": {"plane": n - 2, "flux": outlet}}
 
     def rel(a, b):
         return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
@@ -279,7 +316,7 @@ def precision_table(L, n=192, steps=4000):
         if name == "fp64":
             continue
         out[name] = {"drho": rel(r[0], ref[0]), "u": rel(r[1], ref[1]), "c": rel(r[2], ref[2]),
-                     "outlet_flux": abs(r[3] - ref[3]) / abs(ref[3]), "permeability": abs(r[4] - ref[4]) / abs(ref[4]),
+                     "flux": abs(r[3] - ref[3]) / abs(ref[3]), "permeability": abs(r[4] - ref[4]) / abs(ref[4]),
                      "flow_mass": abs(r[6] - ref[6]) / abs(ref[6]), "solute_mass": abs(r[7] - ref[7]) / abs(ref[7]),
                      "seconds": round(r[5], 3)}
     return out
@@ -394,21 +431,23 @@ def main():
     mask, phi = L.spheres3d(n, n, n, 202, 6.0, 14.0, 0.40, zbuf=4)
     log(f"geometry {n}^3 porosity {phi:.4f} in {time.perf_counter() - t0:.1f}s")
     base = dict(dims=3, ni=n, nj=n, nk=n, tau=(0.8, 0.75), zface_pressure=1, rho_bot=1.001, rho_top=1.0)
+    head_kw = dict(base, nsubs=2, zface_solute=1, c_bot=1.0)
 
-    kw = dict(base, nsubs=1)
-    info, wall, prof, sums = run_workload(L, mask, kw, args.steps, args.warmup, "flow-fp64")
+    # the headline: flow + solute, FP64 storage and collision
+    info, wall, prof, sums = run_workload(L, mask, head_kw, args.steps, args.warmup, "flow-solute-fp64")
     N, Nf = info["nodes"], info["fluid"]
     ms_step = wall / args.steps * 1e3
     mlups = N * args.steps / wall / 1e6
     mflups = Nf * args.steps / wall / 1e6
-    B = algorithmic_bytes(Nf, N, 19, 1, 8)
+    B = algorithmic_bytes(Nf, N, 19, 2, 8)
     odd_ms = prof["odd"][0] / max(prof["odd"][1], 1)
     even_ms = prof["even"][0] / max(prof["even"][1], 1)
     dom = "odd" if odd_ms >= even_ms else "even"
     dom_ms = max(odd_ms, even_ms)
     achieved = B / (dom_ms * 1e-3)
     step_bw = B * args.steps / wall
-    traffic, traffic_src = recorded_traffic(f"k_half_step<19, 1, double, double, {'true' if dom == 'odd' else 'false'}, true, false>")
+    odd_t = "true" if dom == "odd" else "false"
+    traffic, traffic_src = recorded_traffic(f"k_half_step<19, 2, double, double, {odd_t}, true, false>")
     result = {
         "metric": METRIC,
         "value": round(mlups, 2),
@@ -423,13 +462,13 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"d3q19_flow_fp64_{n}cubed_spheres_phi0.40_zouhe",
-            "lattice": [n, n, n], "porosity": round(phi, 5), "fluid_nodes": Nf, "tau": 0.8,
-            "rho_in": 1.001, "rho_out": 1.0, "parallelism": "single-gpu", "geometry_sha256": sha(mask),
+            "workload": f"d3q19_flow_solute_fp64_{n}cubed_spheres_phi0.40_zouhe",
+            "lattice": [n, n, n], "porosity": round(phi, 5), "fluid_nodes": Nf, "tau": 0.8, "tau_s": 0.75,
+            "rho_in": 1.001, "rho_out": 1.0, "c_in": 1.0, "parallelism": "single-gpu",
+            "geometry_sha256": sha(mask),
         },
         "roofline": {
-            "bound": "hbm", "kernel": f"k_half_step<19,1,double,double,{dom == 'odd'}>".replace("True", "true")
-            .replace("False", "false"),
+            "bound": "hbm", "kernel": f"k_half_step<19,2,double,double,{odd_t},true,false>",
             "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic, "traffic_source": traffic_src,
             "bytes_per_launch": B, "avg_launch_ms": round(dom_ms, 5),
@@ -442,23 +481,38 @@ def main():
             "step_frac_wall": round(step_bw / HBM_PEAK, 4),
             "copy_gbps": round(copy_gbps, 1),
             "dominant_frac_of_copy": round(achieved / (copy_gbps * 1e9), 4),
-            "mass_sum": sums[0],
-            # Darcy permeability after warmup + steps (the flow workload is
-            # specified as 5,000 steps), L = LZ - 1 between the face planes
-            "permeability": permeability(sums, N, 0.8, 1.001, 1.0, n - 1),
-            "permeability_after_steps": args.warmup + args.steps,
+            "flow_mass_sum": sums[0], "solute_mass_sum": sums[5],
         },
+    }
+    ext = result["extra"]
+
+    # configuration 2: the flow alone on the same lattice, with its
+    # permeability after the configuration's 5,000 steps from rest (an
+    # untimed continuation when warmup + steps is less)
+    kw_flow = dict(base, nsubs=1)
+    info2, wall2, prof2, sums2 = run_workload(L, mask, kw_flow, args.steps, args.warmup, "flow-fp64",
+                                              total_steps=5000)
+    B2 = algorithmic_bytes(Nf, N, 19, 1, 8)
+    o_ms = prof2["odd"][0] / max(prof2["odd"][1], 1)
+    e_ms = prof2["even"][0] / max(prof2["even"][1], 1)
+    ext["flow_fp64_c2"] = {
+        "mlups": round(N * args.steps / wall2 / 1e6, 1), "mflups": round(Nf * args.steps / wall2 / 1e6, 1),
+        "ms_per_step": round(wall2 / args.steps * 1e3, 4),
+        "odd_frac": round(B2 / (o_ms * 1e-3) / HBM_PEAK, 4), "even_frac": round(B2 / (e_ms * 1e-3) / HBM_PEAK, 4),
+        "step_frac_wall": round(B2 * args.steps / wall2 / HBM_PEAK, 4),
+        "mass_sum": sums2[0],
+        # Darcy permeability, L = LZ - 1 between the face planes
+        "permeability": permeability(sums2, N, 0.8, 1.001, 1.0, n - 1),
+        "permeability_after_steps": info2["steps_from_rest"],
     }
 
     if args.adjoint_only:
-        result["extra"]["adjoint_c4"] = adjoint_timing(L)
+        ext["adjoint_c4"] = adjoint_timing(L)
     elif not args.no_extra:
-        variants = [("flow_solute_fp64", 0, 0, 8), ("flow_solute_fp32_shifted", 1, 1, 4),
-                    ("flow_solute_fp32_shifted_f64_collision", 1, 0, 4), ("flow_solute_fp16_scaled", 2, 1, 2)]
-        ext = {}
+        variants = [("flow_solute_fp32_shifted", 1, 1, 4), ("flow_solute_fp32_shifted_f64_collision", 1, 0, 4),
+                    ("flow_solute_fp16_scaled", 2, 1, 2)]
         for name, st, ct, s in variants:
-            kw2 = dict(base, nsubs=2, zface_solute=1, c_bot=1.0, storage=st, compute=ct,
-                       store_shift_c=0.5 if st == 2 else 0.0)
+            kw2 = dict(head_kw, storage=st, compute=ct)
             steps2 = max(args.steps // 2, 10)
             info2, wall2, prof2, _ = run_workload(L, mask, kw2, steps2, 10, name)
             B2 = algorithmic_bytes(info2["fluid"], info2["nodes"], 19, 2, s)
@@ -475,9 +529,8 @@ def main():
         # Walsh partial bounce-back as the reference's Walsh build runs it:
         # every node computed, the sphere voxels carrying ns = 0.5; the
         # kernel reads the 8-byte ns of each node besides the populations
-        kw2 = dict(base, nsubs=2, zface_solute=1, c_bot=1.0)
         steps2 = max(args.steps // 2, 10)
-        info2, wall2, prof2, _ = run_workload(L, np.zeros_like(mask), kw2, steps2, 10, "walsh",
+        info2, wall2, prof2, _ = run_workload(L, np.zeros_like(mask), head_kw, steps2, 10, "walsh",
                                               ns=np.where(mask == 1, 0.5, 0.0))
         B2 = algorithmic_bytes(info2["fluid"], info2["nodes"], 19, 2, 8) + 8 * info2["fluid"]
         o_ms = prof2["odd"][0] / max(prof2["odd"][1], 1)
@@ -489,26 +542,25 @@ def main():
             "even_frac": round(B2 / (e_ms * 1e-3) / HBM_PEAK, 4),
             "step_frac_wall": round(B2 * steps2 / wall2 / HBM_PEAK, 4),
         }
-        if "flow_solute_fp64" in ext and "flow_solute_fp32_shifted" in ext:
-            ext["fp32_over_fp64_speed"] = round(ext["flow_solute_fp32_shifted"]["mflups"]
-                                                / ext["flow_solute_fp64"]["mflups"], 3)
-        result["extra"].update(ext)
+        if "flow_solute_fp32_shifted" in ext:
+            ext["fp32_over_fp64_speed"] = round(ext["flow_solute_fp32_shifted"]["mflups"] / mflups, 3)
         if not args.variants_only:
-            result["extra"]["adjoint_c4"] = adjoint_timing(L)
-            result["extra"]["throughput_c3"] = throughput_c3(L, cpu_s=0.0 if args.no_cpu else 9.0)
-            result["extra"]["throughput_c1"] = throughput_c1(L, cpu=not args.no_cpu)
-            result["extra"]["precision_c3"] = precision_table(L)
-            result["extra"]["slab_c5"] = slab_c5(L)
+            ext["adjoint_c4"] = adjoint_timing(L)
+            ext["throughput_c3"] = throughput_c3(L, cpu_s=0.0 if args.no_cpu else 9.0)
+            ext["throughput_c1"] = throughput_c1(L, cpu=not args.no_cpu)
+            ext["precision_c3"] = precision_table(L)
+            ext["slab_c5"] = slab_c5(L)
 
     if not args.no_cpu:
-        samples, threads = cpu_baseline(mask, dict(base, nsubs=1), args.cpu_seconds)
+        samples, threads = cpu_baseline(mask, head_kw, args.cpu_seconds)
         rates = sorted(N * n_cpu / el / 1e6 for n_cpu, el in samples)
         med = rates[len(rates) // 2]
         result["cpu_baseline"] = {
             "value": round(med, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"CPU oracle (C, OpenMP, {threads} threads on {os.cpu_count()} visible CPUs, "
+            "sample": f"CPU oracle (the parity build: C, FP64, -O3 -march=x86-64-v3 -ffp-contract=off, OpenMP, "
+                      f"{threads} threads on {os.cpu_count()} visible CPUs, "
                       f"{cpu_model()}, OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}), "
-                      f"same {n}^3 flow FP64 lattice; median of {len(samples)} samples of "
+                      f"same {n}^3 flow + solute FP64 lattice; median of {len(samples)} samples of "
                       + ", ".join(f"{n_cpu} steps in {el:.1f} s" for n_cpu, el in samples),
             "samples_mlups": [round(r, 2) for r in rates],
         }

@@ -92,9 +92,27 @@ typedef struct lbm_deck {
 int lbm_io_read_deck(const char *path, int dims, int nsubs, lbm_deck *deck);
 
 /* The lbm_params a deck describes for the stream-collide library: tau,
- * rho_A, gforce, sink, sigma_t_on/off, the north/south pressure and
- * zero-gradient flags, rho_in/out, pfadd/pfmul and the ramp.  Storage FP64,
- * collision FP64, halo_mode 0, flux_plane -1; series pointers left NULL. */
+ * rho_A, gforce, sink, sigma_t_on/off, rho_in/out, pfadd/pfmul, the ramp
+ * and the boundary flags (storage FP64, collision FP64, halo_mode 0,
+ * flux_plane -1; series pointers left NULL):
+ *   2-D  pressure_{n,s}_{in,out}[s] and zeroconcgrad_{n,s} as the
+ *        reference's GPU path applies them (src/bcs.c:226-557);
+ *        velocity_{n,s}_{in,out}[0] become velocity rows with uy_in/uy_out;
+ *   3-D  the z faces (t = z = LZ-1, b = z = 0; the reference's GPU build
+ *        applies no condition in 3-D, its CPU path has pressure_t_in,
+ *        pressure_b_out, velocity_t_in, velocity_b_out, src/bcs.c:592-862):
+ *        pressure_{t,b}_{in,out}[0] -> a pressure face with rho_in/rho_out,
+ *        velocity_{t,b}_{in,out}[0] -> a velocity face with uz_in/uz_out,
+ *        and with the solute constcon_{t,b}_{in,out} -> a Dirichlet face
+ *        with rho_sigma_in/rho_sigma_out, zeroconcgrad_{t,b} -> a
+ *        zero-gradient face.
+ * Every other boundary flag that is set is an error naming it (east/west
+ * faces, constant-flux faces, zeroconcgrad_full, second-component flow
+ * conditions in 3-D or velocity ones in 2-D, faces of the other dimension,
+ * GZL, the slip and Poiseuille switches), as are two flow or two solute
+ * conditions on one face: the reference's GPU build ignores them silently,
+ * and a deck that sets them expects a condition this build would not
+ * apply. */
 int lbm_io_deck_params(const lbm_deck *deck, int dims, int nsubs, lbm_params *p);
 
 /* Pressure time series ./in/pressure_{n,s}_in0.in (src/latman.c:536-631):

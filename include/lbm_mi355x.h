@@ -68,9 +68,38 @@ typedef struct lbm_params {
                           steps with a macroscopic write; -1 = none */
   int storage;         /* enum lbm_storage */
   int compute;         /* enum lbm_compute */
-  double store_scale[2]; /* FP16: stored = (value - shift) * scale; 0 = default */
-  double store_shift_c;  /* solute storage shift c0: stored g - w_a c0 */
+  double store_scale[2]; /* FP16: stored = (value - shift) * scale; 0 = default
+                            (2^14 flow, 2^12 solute: FP16's normal range then
+                            holds |f~| < 4 and c up to about 40 with 11
+                            significant bits down to 4e-9 and 1.5e-8) */
+  double store_shift_c;  /* solute storage shift c0: stored g - w_a c0 (FP32 and
+                            FP16 storage; 0 keeps small concentrations at full
+                            relative precision) */
+  /* Zou-He velocity rows in 2-D (flow only): the reference parses
+   * velocity_{n,s}_{in,out} and uy_in / uy_out (src/params.h:154-155,
+   * 203-206) but has no kernel for them; here the row's y momentum
+   * rho_A u_y is prescribed (uy_in for *_in, uy_out for *_out, ramped from 0
+   * like the pressure rows' density).  A row may not carry both a pressure
+   * and a velocity condition. */
+  int velocity_n_in, velocity_n_out, velocity_s_in, velocity_s_out;
+  double uy_in, uy_out;
+  /* Per-face 3-D z conditions, [0] the z = 0 face, [1] the z = nk-1 face
+   * (the reference's CPU path: pressure_{t,b}_{in,out} and
+   * velocity_{t,b}_{in,out}, src/bcs.c:592-862).  0 keeps what
+   * zface_pressure / zface_solute select for that face. */
+  int zface_flow[2];   /* enum lbm_zface_flow */
+  double zface_uz[2];  /* u_z of a LBM_ZFACE_VELOCITY face (positive: +z) */
+  int zface_conc[2];   /* enum lbm_zface_conc (solute) */
+  double c_top;        /* concentration of a LBM_ZCONC_DIRICHLET z = nk-1 face
+                          (the z = 0 face takes c_bot and c_bot_map) */
 } lbm_params;
+
+/* flow condition of a z face: pressure takes rho_bot / rho_top, velocity
+ * zface_uz[face] (the face's z momentum rho_A u_z is prescribed; the
+ * incompressible model's velocity is j / rho_A) */
+enum lbm_zface_flow { LBM_ZFACE_DEFAULT = 0, LBM_ZFACE_NONE = 1, LBM_ZFACE_PRESSURE = 2, LBM_ZFACE_VELOCITY = 3 };
+/* solute condition of a z face */
+enum lbm_zface_conc { LBM_ZCONC_DEFAULT = 0, LBM_ZCONC_NONE = 1, LBM_ZCONC_DIRICHLET = 2, LBM_ZCONC_ZEROGRAD = 3 };
 
 /* reductions over the fluid nodes of the last macroscopic write */
 enum lbm_reduction {
@@ -177,6 +206,32 @@ int lbm_reduce(lbm_handle *h, int kind, int subs, double *out);
 /* solute moments of plane flux_plane on the last step with a macroscopic
  * write (or the last step of an objective run) */
 int lbm_last_flux(lbm_handle *h, double *flux, double *mass);
+
+/* Health check of the stored state, the library's form of the reference's
+ * failure handling: checkCUDAError -> process_exit(1) after every launch
+ * (src/lattice.c:3123-3142) and the negative-population check switched by
+ * PUKE_NEGATIVE_DENSITIES ("negative densities generally signify impending
+ * doom", src/flags.h:264-270).  One FP64 pass on the GPU over every stored
+ * population of the fluid nodes fills *out (which may be NULL) and returns
+ * nonzero, with the reason in lbm_last_error(), when
+ *   - any population (flow or solute) is NaN or infinite;
+ *   - any flow population f_a = f~_a + w_a rho_A or density rho_A + delta-rho
+ *     is below zero;
+ *   - mass_ref is given and a distribution's mass has drifted from it by more
+ *     than mass_tol relative: |mass[s] - mass_ref[s]| > mass_tol * M_s with
+ *     M_0 = rho_A N_f + |mass_ref[0]| (the flow's total mass) and
+ *     M_1 = |mass_ref[1]|; a NaN entry of mass_ref skips that distribution.
+ * Negative solute populations are counted but not an error: advection-
+ * diffusion fronts undershoot slightly.  A device error left by an earlier
+ * launch is reported too. */
+typedef struct lbm_health {
+  double mass[2];     /* sum over the fluid nodes of delta-rho (flow) and of c (solute) */
+  double min_rho;     /* least flow density rho_A + delta-rho */
+  double min_pop[2];  /* least finite population: flow f_a, solute g_a */
+  long nonfinite[2];  /* populations that are NaN or infinite */
+  long negative[2];   /* populations below zero */
+} lbm_health;
+int lbm_check(lbm_handle *h, const double *mass_ref, double mass_tol, lbm_health *out);
 
 /* geometry numbers: N, fluid nodes, population slots per direction
  * (fluid nodes and halo ghost entries rounded up to 2048-node tiles) */

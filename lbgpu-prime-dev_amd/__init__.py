@@ -20,6 +20,9 @@ LIB_PATH = os.path.join(_HERE, "lib", "liblbm_mi355x.so")
 
 STORE_F64, STORE_F32, STORE_F16 = 0, 1, 2
 OBJECTIVES = {"flux": 0, "permeability": 1}
+# lbm_zface_flow / lbm_zface_conc
+ZFACE_DEFAULT, ZFACE_NONE, ZFACE_PRESSURE, ZFACE_VELOCITY = 0, 1, 2, 3
+ZCONC_DEFAULT, ZCONC_NONE, ZCONC_DIRICHLET, ZCONC_ZEROGRAD = 0, 1, 2, 3
 COMPUTE_F64, COMPUTE_F32 = 0, 1
 REDUCE_REF, REDUCE_SUMS = 0, 1
 MACRO_NONE, MACRO_FRAME, MACRO_PEST = 0, 1, 2
@@ -28,6 +31,14 @@ PROFILE_CLASSES = ("odd", "even", "bc", "halo", "flux", "adjoint")
 
 class LbmError(RuntimeError):
     pass
+
+
+class Health(ctypes.Structure):
+    """lbm_health: what lbm_check measured"""
+    _fields_ = [
+        ("mass", ctypes.c_double * 2), ("min_rho", ctypes.c_double), ("min_pop", ctypes.c_double * 2),
+        ("nonfinite", ctypes.c_long * 2), ("negative", ctypes.c_long * 2),
+    ]
 
 
 class Params(ctypes.Structure):
@@ -53,6 +64,11 @@ class Params(ctypes.Structure):
         ("flux_plane", ctypes.c_int),
         ("storage", ctypes.c_int), ("compute", ctypes.c_int),
         ("store_scale", ctypes.c_double * 2), ("store_shift_c", ctypes.c_double),
+        ("velocity_n_in", ctypes.c_int), ("velocity_n_out", ctypes.c_int),
+        ("velocity_s_in", ctypes.c_int), ("velocity_s_out", ctypes.c_int),
+        ("uy_in", ctypes.c_double), ("uy_out", ctypes.c_double),
+        ("zface_flow", ctypes.c_int * 2), ("zface_uz", ctypes.c_double * 2),
+        ("zface_conc", ctypes.c_int * 2), ("c_top", ctypes.c_double),
     ]
 
 
@@ -95,6 +111,7 @@ def lib():
             "lbm_sample_macro": [vp, ip, ip, ctypes.POINTER(ctypes.c_long), dp],
             "lbm_reduce": [vp, ip, ip, dp],
             "lbm_last_flux": [vp, dp, dp],
+            "lbm_check": [vp, dp, ctypes.c_double, ctypes.POINTER(Health)],
             "lbm_info": [vp, lp, lp, lp],
             "lbm_profile_enable": [vp, ip],
             "lbm_profile_read": [vp, dp, ctypes.POINTER(ctypes.c_int)],
@@ -162,6 +179,14 @@ def make_params(**kw):
     sc = kw.get("store_scale", (0.0, 0.0))
     p.store_scale[0], p.store_scale[1] = sc[0], sc[1]
     p.store_shift_c = kw.get("store_shift_c", 0.0)
+    for name in ("velocity_n_in", "velocity_n_out", "velocity_s_in", "velocity_s_out"):
+        setattr(p, name, kw.get(name, 0))
+    p.uy_in, p.uy_out = kw.get("uy_in", 0.0), kw.get("uy_out", 0.0)
+    for name in ("zface_flow", "zface_uz", "zface_conc"):
+        v = kw.get(name, (0, 0))
+        arr = getattr(p, name)
+        arr[0], arr[1] = v[0], v[1]
+    p.c_top = kw.get("c_top", 0.0)
     return p
 
 
@@ -323,6 +348,18 @@ class Lattice:
         out = np.empty(6)
         _check(lib().lbm_reduce(self.h, REDUCE_SUMS, 0, _dp(out)))
         return out
+
+    def check(self, mass_ref=None, mass_tol=0.0):
+        """lbm_check: the health of the stored state; raises LbmError with
+        the library's reason when it fails, else returns the lbm_health
+        fields as a dict"""
+        H = Health()
+        ref = None
+        if mass_ref is not None:
+            ref = np.ascontiguousarray(mass_ref, dtype=np.float64).reshape(2)
+        _check(lib().lbm_check(self.h, _dp(ref) if ref is not None else None, float(mass_tol), ctypes.byref(H)))
+        return dict(mass=tuple(H.mass), min_rho=H.min_rho, min_pop=tuple(H.min_pop),
+                    nonfinite=tuple(H.nonfinite), negative=tuple(H.negative))
 
     def last_flux(self):
         a, b = ctypes.c_double(), ctypes.c_double()

@@ -368,8 +368,10 @@ __device__ __forceinline__ void adj_face_block(const AdjArgs &J, long slot, int 
     const AdjOut o = adj_collide(J, m, st);
     double lb[19], mb[19];
     adj_new_all<0>(J, o, k == J.zf ? J.seed : 0.0, J.obs ? J.obs[idx] : 0.0, lam, mu, lb, mb);
-    if (A.zb.flow) adj_zface_flow(top, lb);
-    if (A.zb.solute) {
+    // the adjoint runs with pressure on both faces (or none) and a
+    // Dirichlet inlet with a zero-gradient outlet (or none)
+    if (A.zb.flow[0] != kZNone) adj_zface_flow(top, lb);
+    if (A.zb.conc[0] != kCNone) {
       const double dcb = adj_zface_solute(top, mb);
       if (!top && J.cgrad) J.cgrad[A.G.node[idx]] += dcb;
     }
@@ -521,13 +523,13 @@ __global__ void k_adjoint_zface(AdjBcArgs B, ZfaceArgs Z, double *cgrad) {
     neighbors_rows<3>(B.A.G, i, j, k, r);
 #pragma unroll
     for (int a = 1; a < 19; a++) nb[a] = r[a - 1];
-    if (Z.flow) adj_face<true>(B.L, nb, 0, idx, Z.top, true, dcb);
-    if (Z.solute) adj_face<true>(B.L, nb, 1, idx, Z.top, false, dcb);
+    if (Z.flow != kZNone) adj_face<true>(B.L, nb, 0, idx, Z.top, true, dcb);
+    if (Z.conc != kCNone) adj_face<true>(B.L, nb, 1, idx, Z.top, false, dcb);
   } else {
-    if (Z.flow) adj_face<false>(B.L, nb, 0, idx, Z.top, true, dcb);
-    if (Z.solute) adj_face<false>(B.L, nb, 1, idx, Z.top, false, dcb);
+    if (Z.flow != kZNone) adj_face<false>(B.L, nb, 0, idx, Z.top, true, dcb);
+    if (Z.conc != kCNone) adj_face<false>(B.L, nb, 1, idx, Z.top, false, dcb);
   }
-  if (!Z.top && Z.solute && cgrad) cgrad[B.A.G.node[idx]] += dcb;
+  if (!Z.top && Z.conc != kCNone && cgrad) cgrad[B.A.G.node[idx]] += dcb;
 }
 
 // Observation points of the objective (compact node of each; the

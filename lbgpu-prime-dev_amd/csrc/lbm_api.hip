@@ -55,6 +55,11 @@ struct lbm_handle {
   std::vector<double> series[4];  // n[0], n[1], s[0], s[1]
   std::vector<double> cmap_host;
   int D, Q, U, L;  // dims, directions, unbound directions, last-axis length
+  // the z faces' conditions (3-D), resolved from zface_flow / zface_conc,
+  // zface_pressure and zface_solute: ZFlow / ZConc per face, and the
+  // prescribed values (pressure: rho_b; velocity: rho_A u_z; Dirichlet: c)
+  int zflow[2] = {kZNone, kZNone}, zconc[2] = {kCNone, kCNone};
+  double zfval[2] = {0., 0.}, zcval[2] = {0., 0.};
   long N, E;
   int nf = 0;
   long tiles = 0;  // kTile-node tiles incl. ghost entries
@@ -66,11 +71,13 @@ struct lbm_handle {
   size_t st_bytes = 8;
   void *F = nullptr;
   void *Fout = nullptr;  // where the half-steps write when not F (out-of-place steps)
-  // lbm_run's captured chunks of steps, one per starting parity
+  // lbm_run's captured chunks of steps, one per starting parity and solute
+  // window state (inside the sigma_t_on..sigma_t_off window or not: the
+  // launches differ in solute_on and in the fused z-face conditions)
   unsigned gen = 0;  // bumped when the geometry or the Walsh field changes
-  hipGraphExec_t chunk[2] = {nullptr, nullptr};
-  unsigned chunk_gen[2] = {0, 0};
-  void *chunk_F[2] = {nullptr, nullptr};
+  hipGraphExec_t chunk[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  unsigned chunk_gen[2][2] = {{0, 0}, {0, 0}};
+  void *chunk_F[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   bool chunk_off = false;  // a capture failed: steps are launched one by one
   RowWord *rw = nullptr;
   double *mv = nullptr;
@@ -130,6 +137,9 @@ struct lbm_handle {
 namespace {
 
 constexpr int kRedBlocks = 512;
+// largest LX: the row words hold the ranks of a row's nodes next to a word
+// as 16-bit offsets from the word's base, which are at most LX
+constexpr int kMaxLX = 32767;
 
 hipEvent_t ev_get(lbm_handle *h) {
   if (!h->pool.empty()) {
@@ -196,7 +206,11 @@ StepArgs<ST, CT> make_args(const lbm_handle *h) {
   for (int s = 0; s < 2; s++) {
     double sc = p.store_scale[s];
     if (h->p.storage == LBM_STORE_F16) {
-      if (sc <= 0) sc = s == 0 ? 16384.0 : 16.0;
+      // FP16 keeps 11 significant bits from 6.1e-5 to 65504: 2^14 places
+      // the flow's f~ (|f~| < 4 at the most) and 2^12 the solute's g (c up
+      // to about 40) in that range, so small populations keep their
+      // relative precision down to 4e-9 (flow) and 1.5e-8 (solute)
+      if (sc <= 0) sc = s == 0 ? 16384.0 : 4096.0;
     } else {
       sc = 1.0;
     }
@@ -214,11 +228,14 @@ StepArgs<ST, CT> make_args(const lbm_handle *h) {
 // storing them first.  Not when the plane flux is taken on a face (it reads
 // the stored boundary values) or when the solute condition would write into
 // a resting solute (the separate launch does, and later steps see it).
+bool zface_any(const lbm_handle *h) {
+  return h->zflow[0] != kZNone || h->zflow[1] != kZNone || h->zconc[0] != kCNone || h->zconc[1] != kCNone;
+}
+
 bool zface_fused(const lbm_handle *h, int t) {
   const lbm_params &p = h->p;
-  if (h->D != 3 || !h->zb_ok) return false;
-  const bool sol = p.nsubs == 2 && p.zface_solute;
-  if (!p.zface_pressure && !sol) return false;
+  if (h->D != 3 || !h->zb_ok || !zface_any(h)) return false;
+  const bool sol = h->zconc[0] != kCNone || h->zconc[1] != kCNone;
   if (h->n_flux && (p.flux_plane == 0 || p.flux_plane == p.nk - 1)) return false;
   if (sol && !(t >= p.sigma_t_on && t <= p.sigma_t_off)) return false;
   return true;
@@ -226,16 +243,28 @@ bool zface_fused(const lbm_handle *h, int t) {
 
 template <typename ST, typename CT>
 void set_zb(const lbm_handle *h, StepArgs<ST, CT> &A) {
-  const lbm_params &p = h->p;
   A.zb.on = 1;
   A.zb.lo_end = h->n_lo;
   A.zb.hi_begin = h->nf - h->n_hi;
-  A.zb.flow = p.zface_pressure;
-  A.zb.solute = p.nsubs == 2 && p.zface_solute;
-  A.zb.rho_bot = p.rho_bot;
-  A.zb.rho_top = p.rho_top;
-  A.zb.c_b = p.c_bot;
+  for (int f = 0; f < 2; f++) {
+    A.zb.flow[f] = h->zflow[f];
+    A.zb.conc[f] = h->zconc[f];
+    A.zb.fval[f] = h->zfval[f];
+    A.zb.cval[f] = h->zcval[f];
+  }
   A.zb.cmap = h->cmap;
+}
+
+// the ZfaceArgs of one face for the separate boundary launches
+ZfaceArgs zface_args(const lbm_handle *h, int top) {
+  ZfaceArgs Z;
+  Z.top = top;
+  Z.flow = h->zflow[top];
+  Z.conc = h->zconc[top];
+  Z.fval = h->zfval[top];
+  Z.cval = h->zcval[top];
+  Z.cmap = h->cmap;
+  return Z;
 }
 
 // Calls fn.template operator()<Q, NS, ST, CT>() for the handle's kind.
@@ -284,10 +313,11 @@ void free_objective(lbm_handle *h) {
 }
 
 void drop_chunks(lbm_handle *h) {
-  for (int par = 0; par < 2; par++) {
-    if (h->chunk[par]) (void)hipGraphExecDestroy(h->chunk[par]);
-    h->chunk[par] = nullptr;
-  }
+  for (int par = 0; par < 2; par++)
+    for (int win = 0; win < 2; win++) {
+      if (h->chunk[par][win]) (void)hipGraphExecDestroy(h->chunk[par][win]);
+      h->chunk[par][win] = nullptr;
+    }
   h->gen++;
 }
 
@@ -340,6 +370,23 @@ double bc_rho(const lbm_handle *h, int s, int which, int t) {
     }
   }
   return rho;
+}
+
+// u_y of a 2-D velocity row (uy_in for the *_in flags, uy_out for *_out),
+// with the pressure rows' ramp: 0 before bc_ramp_start, rising linearly to
+// the full value at bc_ramp_stop (which = 0 n_in, 1 n_out, 2 s_in, 3 s_out)
+double bc_uy(const lbm_handle *h, int which, int t) {
+  const lbm_params &p = h->p;
+  double u = (which == 0 || which == 2) ? p.uy_in : p.uy_out;
+  if (p.bc_ramp_start >= 0 && p.bc_ramp_stop >= 0) {
+    if (t < p.bc_ramp_start) {
+      u = 0.0;
+    } else if (t < p.bc_ramp_stop) {
+      const double ramp = ((double)(t - p.bc_ramp_start)) / ((double)(p.bc_ramp_stop - p.bc_ramp_start));
+      u *= ramp;
+    }
+  }
+  return u;
 }
 
 int upload_ints(int **dst, const std::vector<int> &v) {
@@ -464,10 +511,24 @@ int launch_bcs(lbm_handle *h, int phase, int t) {
           const double rb = bc_rho(h, s, c.which, t);
           if (s == 0)
             hipLaunchKernelGGL((k_bc2d_pressure<NS, ST, CT, 0>), dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B,
-                               c.north, rb);
+                               c.north, rb, 0);
           else if constexpr (NS == 2)
             hipLaunchKernelGGL((k_bc2d_pressure<NS, ST, CT, 1>), dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B,
-                               c.north, rb);
+                               c.north, rb, 0);
+        }
+        // velocity rows (flow): the prescribed y momentum rho_A u_y
+        if (s == 0) {
+          struct { int on, north, which; } vr[4] = {{p.velocity_n_in, 1, 0}, {p.velocity_n_out, 1, 1},
+                                                    {p.velocity_s_in, 0, 2}, {p.velocity_s_out, 0, 3}};
+          for (auto &c : vr) {
+            if (!c.on) continue;
+            B.list = c.north ? h->face_hi : h->face_lo;
+            B.n = c.north ? h->n_hi : h->n_lo;
+            if (!B.n) continue;
+            const double mom = p.rho_A[0] * bc_uy(h, c.which, t);
+            hipLaunchKernelGGL((k_bc2d_pressure<NS, ST, CT, 0>), dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B,
+                               c.north, mom, 1);
+          }
         }
         if constexpr (NS == 2) if (s == 1) {
           if (p.zeroconcgrad_n && h->n_hi) {
@@ -483,21 +544,14 @@ int launch_bcs(lbm_handle *h, int phase, int t) {
         }
       }
     } else {
-      int sol = NS == 2 && p.zface_solute;
-      if (p.zface_pressure || sol) {
-        for (int top = 0; top < 2; top++) {
-          ZfaceArgs Z;
-          Z.top = top;
-          Z.flow = p.zface_pressure;
-          Z.solute = sol;
-          Z.rho_b = top ? p.rho_top : p.rho_bot;
-          Z.cmap = h->cmap;
-          Z.c_b = p.c_bot;
-          B.list = top ? h->face_hi : h->face_lo;
-          B.n = top ? h->n_hi : h->n_lo;
-          if (!B.n) continue;
-          hipLaunchKernelGGL((k_bc_zface<NS, ST, CT>), dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B, Z);
-        }
+      for (int top = 0; top < 2; top++) {
+        ZfaceArgs Z = zface_args(h, top);
+        if (NS != 2) Z.conc = kCNone;
+        if (Z.flow == kZNone && Z.conc == kCNone) continue;
+        B.list = top ? h->face_hi : h->face_lo;
+        B.n = top ? h->n_hi : h->n_lo;
+        if (!B.n) continue;
+        hipLaunchKernelGGL((k_bc_zface<NS, ST, CT>), dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B, Z);
       }
     }
     HIP_OK(hipGetLastError());
@@ -597,11 +651,48 @@ extern "C" int lbm_create(const lbm_params *p, lbm_handle **out) {
     return lbm_fail("FP16 storage runs the FP32 collision");
   long N = (long)p->ni * p->nj * (p->dims == 3 ? p->nk : 1);
   if (N >= (1L << 31) - (1L << 24)) return lbm_fail("lattice too large for 32-bit node indices");
+  if (p->ni > kMaxLX) return lbm_fail("LX = %d above %d is not supported", p->ni, kMaxLX);
   if (p->flux_plane >= (p->dims == 2 ? p->nj : p->nk))
     return lbm_fail("flux_plane %d lies outside the lattice", p->flux_plane);
+  for (int f = 0; f < 2; f++) {
+    if (p->zface_flow[f] < LBM_ZFACE_DEFAULT || p->zface_flow[f] > LBM_ZFACE_VELOCITY)
+      return lbm_fail("zface_flow[%d] = %d is not an lbm_zface_flow", f, p->zface_flow[f]);
+    if (p->zface_conc[f] < LBM_ZCONC_DEFAULT || p->zface_conc[f] > LBM_ZCONC_ZEROGRAD)
+      return lbm_fail("zface_conc[%d] = %d is not an lbm_zface_conc", f, p->zface_conc[f]);
+  }
+  if (p->dims == 2) {
+    if ((p->velocity_n_in || p->velocity_n_out) && (p->pressure_n_in[0] || p->pressure_n_out[0]))
+      return lbm_fail("the north row has both a pressure and a velocity condition");
+    if ((p->velocity_s_in || p->velocity_s_out) && (p->pressure_s_in[0] || p->pressure_s_out[0]))
+      return lbm_fail("the south row has both a pressure and a velocity condition");
+  } else if (p->velocity_n_in || p->velocity_n_out || p->velocity_s_in || p->velocity_s_out) {
+    return lbm_fail("velocity rows are 2-D conditions (3-D: zface_flow)");
+  }
   lbm_handle *h = new lbm_handle();
   h->p = *p;
   if (p->dims == 2) h->p.nk = 1;
+  if (p->dims == 3) {
+    // per-face z conditions: explicit kinds, else zface_pressure (both
+    // faces) and zface_solute (Dirichlet inlet at z = 0, zero gradient at
+    // z = nk-1)
+    for (int f = 0; f < 2; f++) {
+      const int fk = p->zface_flow[f];
+      h->zflow[f] = fk == LBM_ZFACE_PRESSURE ? kZPressure
+                    : fk == LBM_ZFACE_VELOCITY ? kZVelocity
+                    : fk == LBM_ZFACE_NONE     ? kZNone
+                    : p->zface_pressure        ? kZPressure
+                                               : kZNone;
+      h->zfval[f] = h->zflow[f] == kZVelocity ? p->rho_A[0] * p->zface_uz[f] : (f ? p->rho_top : p->rho_bot);
+      const int ck = p->zface_conc[f];
+      h->zconc[f] = p->nsubs != 2                 ? kCNone
+                    : ck == LBM_ZCONC_DIRICHLET ? kCDirichlet
+                    : ck == LBM_ZCONC_ZEROGRAD  ? kCZeroGrad
+                    : ck == LBM_ZCONC_NONE      ? kCNone
+                    : p->zface_solute           ? (f ? kCZeroGrad : kCDirichlet)
+                                                : kCNone;
+      h->zcval[f] = f ? p->c_top : p->c_bot;
+    }
+  }
   h->D = p->dims;
   h->Q = p->dims == 2 ? 9 : 19;
   h->U = p->dims == 2 ? 3 : 9;
@@ -660,10 +751,9 @@ extern "C" int lbm_destroy(lbm_handle *h) {
   return 0;
 }
 
-extern "C" int lbm_set_solids(lbm_handle *h, const uint8_t *mask) {
-  if (!h || !mask) return lbm_fail("lbm_set_solids: null argument");
-  HIP_OK(hipStreamSynchronize(h->stream));
-  free_lattice(h);
+// The body of lbm_set_solids on a handle whose lattice has been released;
+// on failure the caller releases whatever it built.
+static int build_lattice(lbm_handle *h, const uint8_t *mask) {
   const long N = h->N, E = h->E;
   const int L = h->L;
   h->fid.assign(N, -1);
@@ -692,10 +782,11 @@ extern "C" int lbm_set_solids(lbm_handle *h, const uint8_t *mask) {
   HIP_OK(hipMemsetAsync(h->mv, 0, (size_t)ns * (1 + h->D) * nf * sizeof(double), h->stream));
   if (upload_ints(&h->node, h->node_host)) return 1;
   // fluid bitmask words with their compact base index, one row at a time
+  // (LX <= kMaxLX, checked by lbm_create, keeps their 16-bit rank offsets
+  // in range)
   const int nx = h->p.ni;
   const long rows = N / nx;
   h->W = (nx + 63) / 64;
-  if (nx > 32767) return lbm_fail("LX above 32767 is not supported");
   std::vector<RowWord> rw((size_t)rows * h->W);
   fill_row_words(h->fid.data(), rows, nx, rw.data());
   HIP_OK(hipMalloc(&h->rw, rw.size() * sizeof(RowWord)));
@@ -744,6 +835,19 @@ extern "C" int lbm_set_solids(lbm_handle *h, const uint8_t *mask) {
     HIP_OK(hipMalloc(&h->halo_neg, hb));
   }
   HIP_OK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+extern "C" int lbm_set_solids(lbm_handle *h, const uint8_t *mask) {
+  if (!h || !mask) return lbm_fail("lbm_set_solids: null argument");
+  HIP_OK(hipStreamSynchronize(h->stream));
+  free_lattice(h);
+  if (int rc = build_lattice(h, mask)) {
+    // no half-built lattice: the handle rejects steps until a later call succeeds
+    (void)hipStreamSynchronize(h->stream);
+    free_lattice(h);
+    return rc;
+  }
   return 0;
 }
 
@@ -882,15 +986,21 @@ bool chunkable(const lbm_handle *h, int t1, int t2) {
 // The HIP graph of kChunk steps from the handle's current time and state
 // array, captured from the step launches on first use and kept while the
 // geometry, the Walsh field and the state array stay the same; null when
-// capture is not possible (the steps are then launched one by one).
+// capture is not possible (the steps are then launched one by one).  One
+// graph per starting parity and per solute window state: chunkable() lets
+// a run through only when all its steps are on the same side of the
+// window, and the graph captured on that side is the one replayed.
 hipGraphExec_t chunk_for(lbm_handle *h) {
   const int par = (h->time + 1) & 1;
-  if (h->chunk[par] && h->chunk_gen[par] == h->gen && h->chunk_F[par] == h->F) return h->chunk[par];
-  if (h->chunk[par]) {
+  const lbm_params &p = h->p;
+  const int win = p.nsubs == 2 && h->time + 1 >= p.sigma_t_on && h->time + 1 <= p.sigma_t_off;
+  hipGraphExec_t &slot = h->chunk[par][win];
+  if (slot && h->chunk_gen[par][win] == h->gen && h->chunk_F[par][win] == h->F) return slot;
+  if (slot) {
     (void)hipStreamSynchronize(h->stream);
-    (void)hipGraphExecDestroy(h->chunk[par]);
+    (void)hipGraphExecDestroy(slot);
   }
-  h->chunk[par] = nullptr;
+  slot = nullptr;
   const int t_save = h->time;
   hipGraph_t g = nullptr;
   if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
@@ -910,9 +1020,9 @@ hipGraphExec_t chunk_for(lbm_handle *h) {
     return nullptr;
   }
   (void)hipGraphDestroy(g);
-  h->chunk[par] = x;
-  h->chunk_gen[par] = h->gen;
-  h->chunk_F[par] = h->F;
+  slot = x;
+  h->chunk_gen[par][win] = h->gen;
+  h->chunk_F[par][win] = h->F;
   return x;
 }
 
@@ -1104,6 +1214,60 @@ extern "C" int lbm_last_flux(lbm_handle *h, double *flux, double *mass) {
   return 0;
 }
 
+extern "C" int lbm_check(lbm_handle *h, const double *mass_ref, double mass_tol, lbm_health *out) {
+  if (need_lattice(h)) return 1;
+  // an error left on the device by an earlier launch is a failure too
+  HIP_OK(hipStreamSynchronize(h->stream));
+  HIP_OK(hipGetLastError());
+  int rc = dispatch(h, [&]<int Q, int NS, typename ST, typename CT>() -> int {
+    StepArgs<ST, CT> A = make_args<ST, CT>(h);
+    hipLaunchKernelGGL((k_check<Q, NS, ST, CT>), dim3(kRedBlocks), dim3(256), 0, h->stream, A, h->p.rho_A[0],
+                       h->red);
+    HIP_OK(hipGetLastError());
+    return 0;
+  });
+  if (rc) return rc;
+  std::vector<double> part((size_t)kCheckVals * kRedBlocks);
+  HIP_OK(hipMemcpyAsync(part.data(), h->red, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  double v[kCheckVals];
+  for (int q = 0; q < kCheckVals; q++) v[q] = part[q];
+  for (int b = 1; b < kRedBlocks; b++)
+    for (int q = 0; q < kCheckVals; q++) {
+      const double x = part[(size_t)b * kCheckVals + q];
+      v[q] = (q >= 2 && q <= 4) ? fmin(v[q], x) : v[q] + x;
+    }
+  lbm_health H;
+  H.mass[0] = v[0];
+  H.mass[1] = h->p.nsubs == 2 ? v[1] : 0.0;
+  H.min_rho = v[2];
+  H.min_pop[0] = v[3];
+  H.min_pop[1] = h->p.nsubs == 2 ? v[4] : 0.0;
+  for (int s = 0; s < 2; s++) {
+    H.nonfinite[s] = (long)v[5 + s];
+    H.negative[s] = (long)v[7 + s];
+  }
+  if (out) *out = H;
+  static const char *name[2] = {"flow", "solute"};
+  for (int s = 0; s < h->p.nsubs; s++)
+    if (H.nonfinite[s])
+      return lbm_fail("health check: %ld %s populations are NaN or infinite", H.nonfinite[s], name[s]);
+  if (H.negative[0] || !(H.min_rho > 0.0))
+    return lbm_fail("health check: %ld negative flow populations (least %g), least density %g", H.negative[0],
+                    H.min_pop[0], H.min_rho);
+  if (mass_ref) {
+    for (int s = 0; s < h->p.nsubs; s++) {
+      if (mass_ref[s] != mass_ref[s]) continue;  // NaN: not checked
+      const double M = s == 0 ? h->p.rho_A[0] * (double)h->nf + fabs(mass_ref[0]) : fabs(mass_ref[1]);
+      const double d = fabs(H.mass[s] - mass_ref[s]);
+      if (!(d <= mass_tol * M))
+        return lbm_fail("health check: %s mass %.17g drifted from %.17g (relative %.3g > %g)", name[s], H.mass[s],
+                        mass_ref[s], M > 0.0 ? d / M : d, mass_tol);
+    }
+  }
+  return 0;
+}
+
 extern "C" int lbm_info(const lbm_handle *h, long *n_nodes, long *n_fluid, long *row_stride) {
   if (!h) return lbm_fail("null handle");
   if (n_nodes) *n_nodes = h->N;
@@ -1182,6 +1346,14 @@ static int adjoint_supported(const lbm_handle *h) {
   if (h->obj.kind == LBM_OBJ_OUTLET_FLUX && (p.flux_plane < 0 || p.flux_plane >= p.nk))
     return lbm_fail("the objective needs flux_plane");
   if (h->ns) return lbm_fail("the adjoint does not cover partial bounce-back (lbm_set_ns)");
+  // the transposed face conditions: pressure on both z faces or on none; a
+  // Dirichlet inlet at z = 0 with a zero-gradient outlet, or no solute
+  // condition
+  if (h->zflow[0] != h->zflow[1] || h->zflow[0] == kZVelocity)
+    return lbm_fail("the adjoint needs the z-face pressure condition on both faces or on none");
+  if (!((h->zconc[0] == kCDirichlet && h->zconc[1] == kCZeroGrad) ||
+        (h->zconc[0] == kCNone && h->zconc[1] == kCNone)))
+    return lbm_fail("the adjoint needs a Dirichlet solute inlet at z = 0 with a zero-gradient outlet, or none");
   return 0;
 }
 
@@ -1561,8 +1733,7 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
         break;
       }
       if (fz) continue;
-      const lbm_params &p = h->p;
-      if (p.zface_pressure || p.zface_solute) {
+      if (zface_any(h)) {
         for (int top = 0; top < 2; top++) {
           AdjBcArgs B;
           B.A = J.A;
@@ -1571,13 +1742,7 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
           B.n = top ? h->n_hi : h->n_lo;
           B.phase = odd ? 1 : 2;
           if (!B.n) continue;
-          ZfaceArgs Z;
-          Z.top = top;
-          Z.flow = p.zface_pressure;
-          Z.solute = p.zface_solute;
-          Z.rho_b = top ? p.rho_top : p.rho_bot;
-          Z.cmap = h->cmap;
-          Z.c_b = p.c_bot;
+          const ZfaceArgs Z = zface_args(h, top);
           hipLaunchKernelGGL(k_adjoint_zface, dim3(blocks_for(B.n)), dim3(256), 0, h->stream, B, Z, cgrad);
           if ((e = hipGetLastError()) != hipSuccess) fail("adjoint BC", e);
         }

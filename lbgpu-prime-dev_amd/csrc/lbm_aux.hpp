@@ -164,6 +164,45 @@ __global__ __launch_bounds__(256) void k_reduce_mv(const double *mv, int nf, dou
   block_reduce<12>(v, kind, partial + (long)blockIdx.x * 12);
 }
 
+// Health of the stored state (lbm_check), over every stored population of
+// the fluid nodes, per block: [0] sum of delta-rho, [1] sum of c, [2] least
+// density rho_A + delta-rho, [3] least flow population f_a = f~_a + w_a rho_A,
+// [4] least solute population g_a, [5] / [6] non-finite flow / solute
+// populations, [7] / [8] negative flow / solute populations.  The counts are
+// carried as doubles (exact below 2^53).
+constexpr int kCheckVals = 9;
+
+template <int Q, int NS, typename ST, typename CT>
+__global__ __launch_bounds__(256) void k_check(StepArgs<ST, CT> A, double rho_A, double *partial) {
+  double v[kCheckVals];
+  const int kind[kCheckVals] = {0, 0, 2, 2, 2, 0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < kCheckVals; q++) v[q] = kind[q] == 2 ? __builtin_huge_val() : 0.0;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < A.nf; idx += gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      double m = 0.0;
+#pragma unroll
+      for (int a = 0; a < Q; a++) {
+        const ST raw = A.F[foff<Q, NS>(s, a, idx)];
+        const double x = s == 0 ? (double)Codec<ST, CT>::dec(raw, A.inv_scale[0], (CT)0)
+                                : (double)Codec<ST, CT>::dec(raw, A.inv_scale[1], shift_of<Q, ST, CT, 1>(A, a));
+        const double pop = s == 0 ? x + W<Q>(a) * rho_A : x;
+        if (!__builtin_isfinite(x)) {
+          v[5 + s] += 1.0;
+        } else {
+          if (pop < 0.0) v[7 + s] += 1.0;
+          v[3 + s] = fmin(v[3 + s], pop);
+        }
+        m += x;
+      }
+      v[s] += m;
+      if (s == 0 && __builtin_isfinite(m)) v[2] = fmin(v[2], rho_A + m);
+    }
+  }
+  block_reduce<kCheckVals>(v, kind, partial + (long)blockIdx.x * kCheckVals);
+}
+
 // solute moments of one plane: sum_a c_a,last g_a and sum_a g_a on the
 // populations the coming half-step collides.  Stage 1: kFluxBlocks blocks
 // stride over the plane's nodes and leave per-block partial sums; stage 2:

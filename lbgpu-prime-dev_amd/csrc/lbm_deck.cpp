@@ -359,6 +359,10 @@ int deck_main(int argc, char **argv) {
       opos++;
     }
     if (!end_of_frame) continue;
+    // at every frame: no device error, no NaN/infinite population, no
+    // negative flow population (checkCUDAError and PUKE_NEGATIVE_DENSITIES:
+    // the run stops with exit status 1)
+    check(lbm_check(h, nullptr, 0.0, nullptr), "health check");
     auto a = std::chrono::steady_clock::now();
     ++frame;
     for (int s = 0; s < R.nsubs; s++) {

@@ -102,6 +102,12 @@ __device__ __forceinline__ void st_stream(T *p, T v) {
   else *p = v;
 }
 
+// Kinds of the 3-D z-face conditions, one per face ([0] z = 0, [1] z = nk - 1).
+// Flow: Zou-He with a prescribed density (pressure) or z velocity; solute:
+// Dirichlet concentration or zero gradient.
+enum ZFlow : int { kZNone = 0, kZPressure = 1, kZVelocity = 2 };
+enum ZConc : int { kCNone = 0, kCDirichlet = 1, kCZeroGrad = 2 };
+
 // 3-D z-face conditions applied by the half-step itself to the populations
 // a face node has gathered (the values k_bc_zface would have stored where
 // the node reads them).  Compact indices [0, lo_end) are the fluid nodes
@@ -109,9 +115,11 @@ __device__ __forceinline__ void st_stream(T *p, T v) {
 struct ZfaceFuse {
   int on;
   int lo_end, hi_begin;
-  int flow, solute;
-  double rho_bot, rho_top, c_b;
-  const double *cmap;  // [nj][ni] inlet concentration or null
+  int flow[2];         // ZFlow per face
+  int conc[2];         // ZConc per face
+  double fval[2];      // pressure face: rho_b; velocity face: rho_A u_z, its z momentum
+  double cval[2];      // Dirichlet concentration
+  const double *cmap;  // [nj][ni] Dirichlet concentration of the z = 0 face, or null
 };
 
 template <typename ST, typename CT>
@@ -372,11 +380,15 @@ __device__ __forceinline__ void collide(const StepArgs<ST, CT> &A, int idx, CT (
 constexpr int kUp[5] = {9, 11, 13, 15, 17};
 constexpr int kDn[5] = {10, 12, 14, 16, 18};
 
-// Zou-He pressure with zero tangential velocity (Hecht & Harting's D3Q19
-// form): the five unknowns of a face node from its known populations and
-// the prescribed density offset drb = rho_b - rho_A.
+// Zou-He with zero tangential velocity (Hecht & Harting's D3Q19 form): the
+// five unknowns of a face node from its known populations and either the
+// prescribed density offset val = rho_b - rho_A (pressure: the node's z
+// momentum J follows from the mass balance) or the prescribed z momentum
+// val = rho_A u_z (velocity: the incompressible model's u = j / rho_A, the
+// density then follows from the unknowns).
 template <typename CT>
-__device__ __forceinline__ void zface_flow_values(const CT (&f)[19], bool top, CT drb, CT (&v)[5]) {
+__device__ __forceinline__ void zface_flow_values(const CT (&f)[19], bool top, bool velocity, CT val,
+                                                  CT (&v)[5]) {
   CT P = f[0];
 #pragma unroll
   for (int a = 1; a < 9; a++) P += f[a];
@@ -393,14 +405,14 @@ __device__ __forceinline__ void zface_flow_values(const CT (&f)[19], bool top, C
   const CT Nx = ((f[1] + f[5] + f[8]) - (f[2] + f[7] + f[6])) / (CT)2;
   const CT Ny = ((f[3] + f[5] + f[7]) - (f[4] + f[6] + f[8])) / (CT)2;
   if (!top) {
-    const CT J = drb - (P + (CT)2 * M);
+    const CT J = velocity ? val : val - (P + (CT)2 * M);
     v[0] = f[10] + J / (CT)3;
     v[1] = f[12] + J / (CT)6 - Nx;
     v[2] = f[14] + J / (CT)6 + Nx;
     v[3] = f[16] + J / (CT)6 - Ny;
     v[4] = f[18] + J / (CT)6 + Ny;
   } else {
-    const CT J = (P + (CT)2 * M) - drb;
+    const CT J = velocity ? val : (P + (CT)2 * M) - val;
     v[0] = f[9] - J / (CT)3;
     v[1] = f[11] - J / (CT)6 + Nx;
     v[2] = f[13] - J / (CT)6 - Nx;
@@ -409,26 +421,40 @@ __device__ __forceinline__ void zface_flow_values(const CT (&f)[19], bool top, C
   }
 }
 
-// Solute: Dirichlet inlet (bottom, c = cb: unknowns w_u c* with c* from the
-// known populations) and zero-gradient outlet (top, c* from the up-movers).
+// Solute: the unknowns are w_u c*.  Dirichlet (c = cb): c* = (cb - sum of
+// the 14 known populations) / (sum of the unknowns' weights, 1/6).  Zero
+// gradient: c* = (sum of the five outgoing populations) / (1/6), the 3-D
+// form of k_sysbound_zeroconcgrad_n.  Bottom face: the unknowns move up,
+// the outgoing populations down; top face the other way round.
 template <typename CT>
-__device__ __forceinline__ void zface_solute_values(const CT (&g)[19], bool top, CT cb, CT (&v)[5]) {
+__device__ __forceinline__ void zface_solute_values(const CT (&g)[19], bool top, bool dirichlet, CT cb,
+                                                    CT (&v)[5]) {
   CT known = (CT)0, cs;
-  if (!top) {
+  if (dirichlet) {
 #pragma unroll
     for (int a = 0; a < 9; a++) known += g[a];
+  }
+  if (!top) {
 #pragma unroll
     for (int q = 0; q < 5; q++) known += g[kDn[q]];
-    cs = (cb - known) / (CT)(1. / 6.);
+    cs = dirichlet ? (cb - known) / (CT)(1. / 6.) : known / (CT)(1. / 6.);
 #pragma unroll
     for (int q = 0; q < 5; q++) v[q] = (CT)W<19>(kUp[q]) * cs;
   } else {
 #pragma unroll
     for (int q = 0; q < 5; q++) known += g[kUp[q]];
-    cs = known / (CT)(1. / 6.);
+    cs = dirichlet ? (cb - known) / (CT)(1. / 6.) : known / (CT)(1. / 6.);
 #pragma unroll
     for (int q = 0; q < 5; q++) v[q] = (CT)W<19>(kDn[q]) * cs;
   }
+}
+
+// the prescribed value of a face's flow condition in the compute type:
+// rho_b - rho_A formed in FP64 (1.001 rounded to FP32 first would move a
+// 1e-3 offset by 5e-5), or the z momentum rho_A u_z formed on the host
+template <typename CT>
+__device__ __forceinline__ CT zface_flow_val(int kind, double fval, CT rhoA) {
+  return kind == kZVelocity ? (CT)fval : (CT)(fval - (double)rhoA);
 }
 
 // put the five values into the unknown slots of the gathered array, through
@@ -456,17 +482,21 @@ __device__ __forceinline__ bool zface_node(const StepArgs<ST, CT> &A, int idx) {
 template <typename ST, typename CT, bool SOLUTE>
 __device__ __forceinline__ void zface_fix(const StepArgs<ST, CT> &A, int idx, CT (&f)[19], CT (&g)[19]) {
   const bool top = idx >= A.zb.hi_begin;
+  const int fk = top ? A.zb.flow[1] : A.zb.flow[0];
   CT v[5];
-  if (A.zb.flow) {
-    // the offset rho_b - rho_A formed in FP64 (the same expression in FP64)
-    zface_flow_values<CT>(f, top, (CT)((top ? A.zb.rho_top : A.zb.rho_bot) - (double)A.rhoA), v);
+  if (fk != kZNone) {
+    zface_flow_values<CT>(f, top, fk == kZVelocity, zface_flow_val<CT>(fk, top ? A.zb.fval[1] : A.zb.fval[0], A.rhoA),
+                          v);
     zface_put<ST, CT, 0>(A, top, v, f);
   }
-  if (SOLUTE && A.zb.solute) {
-    CT cb = (CT)A.zb.c_b;
-    if (!top && A.zb.cmap) cb = (CT)A.zb.cmap[A.G.node[idx]];  // z = 0: dense index = i + ni j
-    zface_solute_values<CT>(g, top, cb, v);
-    zface_put<ST, CT, 1>(A, top, v, g);
+  if (SOLUTE) {
+    const int ck = top ? A.zb.conc[1] : A.zb.conc[0];
+    if (ck != kCNone) {
+      CT cb = (CT)(top ? A.zb.cval[1] : A.zb.cval[0]);
+      if (!top && A.zb.cmap) cb = (CT)A.zb.cmap[A.G.node[idx]];  // z = 0: dense index = i + ni j
+      zface_solute_values<CT>(g, top, ck == kCDirichlet, cb, v);
+      zface_put<ST, CT, 1>(A, top, v, g);
+    }
   }
 }
 
@@ -582,8 +612,16 @@ __device__ __forceinline__ void bc_store(const BcArgs<ST, CT> &B, int idx, const
 // The prescribed density arrives in FP64: in the FP32 collision the
 // offset rho_b - rho_A is formed in FP64 first (1.001 rounded to FP32 would
 // move a 1e-3 offset by 5e-5), in FP64 the reference's order is kept.
+// With velocity set (the velocity_{n,s}_{in,out} flags, which the
+// reference parses but never applies) the row's y momentum x = rho_A u_y is
+// prescribed instead (val = rho_A u_y, formed on the host) and the same
+// three unknowns follow from it; having no reference kernel to follow, the
+// velocity rows place an unknown whose source is solid in the own slot
+// opp(a), where the pull reads it (as the 3-D faces do), instead of the
+// reference rows' own slot a (which leaves the bounced value in place and
+// hands the unknown to the interior node beside a side wall).
 template <int NS, typename ST, typename CT, int SUBS>
-__global__ void k_bc2d_pressure(BcArgs<ST, CT> B, int north, double rho_b) {
+__global__ void k_bc2d_pressure(BcArgs<ST, CT> B, int north, double rho_b, int velocity) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= B.n) return;
   const int idx = B.list[t];
@@ -594,7 +632,9 @@ __global__ void k_bc2d_pressure(BcArgs<ST, CT> B, int north, double rho_b) {
   const CT rhoA = SUBS == 0 ? B.A.rhoA : B.A.rhoA_s;
   if (north) {
     CT x = f[C_] + f[E_] + f[W_] + f[N_] + f[N_] + f[NE_] + f[NE_] + f[NW_] + f[NW_];
-    if constexpr (sizeof(CT) == 4) {
+    if (velocity) {
+      x = (CT)rho_b;
+    } else if constexpr (sizeof(CT) == 4) {
       x -= (CT)(rho_b - (double)rhoA);
     } else {
       x -= (CT)rho_b;
@@ -603,12 +643,14 @@ __global__ void k_bc2d_pressure(BcArgs<ST, CT> B, int north, double rho_b) {
     CT vS = f[N_] - ((CT)(2. / 3.)) * x;
     CT vSW = f[NE_] + ((CT)(1. / 2.)) * (f[E_] - f[W_]) - ((CT)(1. / 6.)) * x;
     CT vSE = f[NW_] + ((CT)(1. / 2.)) * (f[W_] - f[E_]) - ((CT)(1. / 6.)) * x;
-    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, S_, vS, false);
-    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, SE_, vSE, false);
-    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, SW_, vSW, false);
+    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, S_, vS, velocity);
+    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, SE_, vSE, velocity);
+    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, SW_, vSW, velocity);
   } else {
     CT x = -f[C_] - f[E_] - f[W_] - f[S_] - f[S_] - f[SE_] - f[SE_] - f[SW_] - f[SW_];
-    if constexpr (sizeof(CT) == 4) {
+    if (velocity) {
+      x = (CT)rho_b;
+    } else if constexpr (sizeof(CT) == 4) {
       x += (CT)(rho_b - (double)rhoA);
     } else {
       x += (CT)rho_b;
@@ -617,9 +659,9 @@ __global__ void k_bc2d_pressure(BcArgs<ST, CT> B, int north, double rho_b) {
     CT vN = f[S_] + ((CT)(2. / 3.)) * x;
     CT vNE = f[SW_] + ((CT)(1. / 2.)) * (f[W_] - f[E_]) + ((CT)(1. / 6.)) * x;
     CT vNW = f[SE_] + ((CT)(1. / 2.)) * (f[E_] - f[W_]) + ((CT)(1. / 6.)) * x;
-    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, N_, vN, false);
-    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, NW_, vNW, false);
-    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, NE_, vNE, false);
+    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, N_, vN, velocity);
+    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, NW_, vNW, velocity);
+    bc_store<9, NS, ST, CT, SUBS>(B, idx, nb, NE_, vNE, velocity);
   }
 }
 
@@ -648,17 +690,19 @@ __global__ void k_bc2d_zerograd(BcArgs<ST, CT> B, int north) {
   }
 }
 
-// 3-D z faces (no reference counterpart).  Flow: Zou-He pressure with zero
-// tangential velocity in the D3Q19 form of Hecht & Harting (J. Stat. Mech.
-// 2010, P01018) on shifted populations; solute: Dirichlet c at z=0, zero
-// gradient at z=nk-1.  Unknowns replace what streaming would deliver.
+// 3-D z faces (the reference's GPU path has none; its CPU path has the
+// pressure and velocity forms on the top and bottom faces, src/bcs.c:592-862).
+// Flow: Zou-He pressure or velocity with zero tangential velocity in the
+// D3Q19 form of Hecht & Harting (J. Stat. Mech. 2010, P01018) on shifted
+// populations; solute: Dirichlet concentration or zero gradient.  Unknowns
+// replace what streaming would deliver.  One launch per face.
 struct ZfaceArgs {
   int top;
-  int flow;
-  int solute;
-  double rho_b;       // prescribed density of this face
-  const double *cmap; // [nj][ni] inlet concentration or null
-  double c_b;
+  int flow;           // ZFlow of this face
+  int conc;           // ZConc of this face (solute)
+  double fval;        // pressure: rho_b; velocity: rho_A u_z
+  double cval;        // Dirichlet concentration
+  const double *cmap; // [nj][ni] Dirichlet concentration of the z = 0 face, or null
 };
 
 template <int NS, typename ST, typename CT>
@@ -669,9 +713,9 @@ __global__ void k_bc_zface(BcArgs<ST, CT> B, ZfaceArgs Z) {
   int nb[19];
   CT f[19], v[5];
   const bool top = Z.top;
-  if (Z.flow) {
+  if (Z.flow != kZNone) {
     bc_gather<19, NS, ST, CT, 0>(B, idx, nb, f);
-    zface_flow_values<CT>(f, top, (CT)(Z.rho_b - (double)B.A.rhoA), v);
+    zface_flow_values<CT>(f, top, Z.flow == kZVelocity, zface_flow_val<CT>(Z.flow, Z.fval, B.A.rhoA), v);
     if (top) {
 #pragma unroll
       for (int q = 0; q < 5; q++) bc_store<19, NS, ST, CT, 0>(B, idx, nb, kDn[q], v[q], true);
@@ -680,11 +724,11 @@ __global__ void k_bc_zface(BcArgs<ST, CT> B, ZfaceArgs Z) {
       for (int q = 0; q < 5; q++) bc_store<19, NS, ST, CT, 0>(B, idx, nb, kUp[q], v[q], true);
     }
   }
-  if (NS == 2 && Z.solute) {
+  if (NS == 2 && Z.conc != kCNone) {
     bc_gather<19, NS, ST, CT, 1>(B, idx, nb, f);
-    CT cb = (CT)Z.c_b;
+    CT cb = (CT)Z.cval;
     if (!top && Z.cmap) cb = (CT)Z.cmap[B.A.G.node[idx]];  // z = 0: dense index = i + ni j
-    zface_solute_values<CT>(f, top, cb, v);
+    zface_solute_values<CT>(f, top, Z.conc == kCDirichlet, cb, v);
     if (top) {
 #pragma unroll
       for (int q = 0; q < 5; q++) bc_store<19, NS, ST, CT, 1>(B, idx, nb, kDn[q], v[q], true);

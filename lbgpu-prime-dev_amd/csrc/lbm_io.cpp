@@ -217,16 +217,12 @@ extern "C" int lbm_io_deck_params(const lbm_deck *d, int dims, int nsubs, lbm_pa
     p->tau[s] = s < nsubs ? d->tau[s] : 1.0;
     p->rho_A[s] = d->rho_A[s];
     for (int c = 0; c < 3; c++) p->gaccel[3 * s + c] = s < nsubs ? d->gforce[s][c] : 0.0;
-    p->pressure_n_in[s] = d->pressure_n_in[s];
-    p->pressure_n_out[s] = d->pressure_n_out[s];
-    p->pressure_s_in[s] = d->pressure_s_in[s];
-    p->pressure_s_out[s] = d->pressure_s_out[s];
   }
   p->sink = d->sink;
   p->sigma_t_on = d->sigma_t_on;
   p->sigma_t_off = d->sigma_t_off;
-  p->zeroconcgrad_n = d->zeroconcgrad_n;
-  p->zeroconcgrad_s = d->zeroconcgrad_s;
+  p->zeroconcgrad_n = nsubs == 2 ? d->zeroconcgrad_n : 0;
+  p->zeroconcgrad_s = nsubs == 2 ? d->zeroconcgrad_s : 0;
   p->rho_in = d->rho_in;
   p->rho_out = d->rho_out;
   p->pfadd = d->pfadd;
@@ -236,6 +232,159 @@ extern "C" int lbm_io_deck_params(const lbm_deck *d, int dims, int nsubs, lbm_pa
   p->flux_plane = -1;
   p->storage = LBM_STORE_F64;
   p->compute = LBM_COMPUTE_F64;
+
+  // flags no kernel of this build applies
+  struct Flag {
+    const char *name;
+    int v;
+  };
+  std::vector<Flag> bad;
+  auto no = [&](const char *name, int v) {
+    if (v) bad.push_back({name, v});
+  };
+  for (int s = 0; s < 2; s++) {
+    const bool two = s == 1;
+    auto nm = [&](const char *a, const char *b) { return two ? b : a; };
+    no(nm("pressure_e_in[0]", "pressure_e_in[1]"), d->pressure_e_in[s]);
+    no(nm("pressure_w_in[0]", "pressure_w_in[1]"), d->pressure_w_in[s]);
+    no(nm("pressure_e_out[0]", "pressure_e_out[1]"), d->pressure_e_out[s]);
+    no(nm("pressure_w_out[0]", "pressure_w_out[1]"), d->pressure_w_out[s]);
+    no(nm("velocity_e_in[0]", "velocity_e_in[1]"), d->velocity_e_in[s]);
+    no(nm("velocity_w_in[0]", "velocity_w_in[1]"), d->velocity_w_in[s]);
+    no(nm("velocity_e_out[0]", "velocity_e_out[1]"), d->velocity_e_out[s]);
+    no(nm("velocity_w_out[0]", "velocity_w_out[1]"), d->velocity_w_out[s]);
+    if (dims == 2) {
+      no(nm("pressure_t_in[0]", "pressure_t_in[1]"), d->pressure_t_in[s]);
+      no(nm("pressure_b_in[0]", "pressure_b_in[1]"), d->pressure_b_in[s]);
+      no(nm("pressure_t_out[0]", "pressure_t_out[1]"), d->pressure_t_out[s]);
+      no(nm("pressure_b_out[0]", "pressure_b_out[1]"), d->pressure_b_out[s]);
+      no(nm("velocity_t_in[0]", "velocity_t_in[1]"), d->velocity_t_in[s]);
+      no(nm("velocity_b_in[0]", "velocity_b_in[1]"), d->velocity_b_in[s]);
+      no(nm("velocity_t_out[0]", "velocity_t_out[1]"), d->velocity_t_out[s]);
+      no(nm("velocity_b_out[0]", "velocity_b_out[1]"), d->velocity_b_out[s]);
+    } else {
+      no(nm("pressure_n_in[0]", "pressure_n_in[1]"), d->pressure_n_in[s]);
+      no(nm("pressure_s_in[0]", "pressure_s_in[1]"), d->pressure_s_in[s]);
+      no(nm("pressure_n_out[0]", "pressure_n_out[1]"), d->pressure_n_out[s]);
+      no(nm("pressure_s_out[0]", "pressure_s_out[1]"), d->pressure_s_out[s]);
+    }
+    if (dims == 3 || two) {
+      no(nm("velocity_n_in[0]", "velocity_n_in[1]"), d->velocity_n_in[s]);
+      no(nm("velocity_s_in[0]", "velocity_s_in[1]"), d->velocity_s_in[s]);
+      no(nm("velocity_n_out[0]", "velocity_n_out[1]"), d->velocity_n_out[s]);
+      no(nm("velocity_s_out[0]", "velocity_s_out[1]"), d->velocity_s_out[s]);
+    }
+    if (dims == 3 && two) {
+      no("pressure_t_in[1]", d->pressure_t_in[1]);
+      no("pressure_b_in[1]", d->pressure_b_in[1]);
+      no("pressure_t_out[1]", d->pressure_t_out[1]);
+      no("pressure_b_out[1]", d->pressure_b_out[1]);
+      no("velocity_t_in[1]", d->velocity_t_in[1]);
+      no("velocity_b_in[1]", d->velocity_b_in[1]);
+      no("velocity_t_out[1]", d->velocity_t_out[1]);
+      no("velocity_b_out[1]", d->velocity_b_out[1]);
+    }
+  }
+  no("constflx_t_in", d->constflx_t_in);
+  no("constflx_b_in", d->constflx_b_in);
+  no("constflx_t_out", d->constflx_t_out);
+  no("constflx_b_out", d->constflx_b_out);
+  no("constflx_n_in", d->constflx_n_in);
+  no("constflx_s_in", d->constflx_s_in);
+  no("constflx_n_out", d->constflx_n_out);
+  no("constflx_s_out", d->constflx_s_out);
+  no("constflx_e_in", d->constflx_e_in);
+  no("constflx_w_in", d->constflx_w_in);
+  no("constflx_e_out", d->constflx_e_out);
+  no("constflx_w_out", d->constflx_w_out);
+  no("constcon_n_in", d->constcon_n_in);
+  no("constcon_s_in", d->constcon_s_in);
+  no("constcon_n_out", d->constcon_n_out);
+  no("constcon_s_out", d->constcon_s_out);
+  no("constcon_e_in", d->constcon_e_in);
+  no("constcon_w_in", d->constcon_w_in);
+  no("constcon_e_out", d->constcon_e_out);
+  no("constcon_w_out", d->constcon_w_out);
+  no("zeroconcgrad_e", d->zeroconcgrad_e);
+  no("zeroconcgrad_w", d->zeroconcgrad_w);
+  no("zeroconcgrad_full", d->zeroconcgrad_full);
+  if (dims == 2) {
+    no("constcon_t_in", d->constcon_t_in);
+    no("constcon_b_in", d->constcon_b_in);
+    no("constcon_t_out", d->constcon_t_out);
+    no("constcon_b_out", d->constcon_b_out);
+    no("zeroconcgrad_t", d->zeroconcgrad_t);
+    no("zeroconcgrad_b", d->zeroconcgrad_b);
+  } else {
+    no("zeroconcgrad_n", d->zeroconcgrad_n);
+    no("zeroconcgrad_s", d->zeroconcgrad_s);
+  }
+  no("GZL", d->GZL);
+  no("bc_poisseuille", d->bc_poisseuille);
+  no("bc_slip_north", d->bc_slip_north);
+  no("bc_sigma_slip", d->bc_sigma_slip);
+  if (!bad.empty())
+    return lbm_fail("%s = %d is set, and this build has no such boundary condition", bad[0].name, bad[0].v);
+
+  if (dims == 2) {
+    p->velocity_n_in = d->velocity_n_in[0];
+    p->velocity_n_out = d->velocity_n_out[0];
+    p->velocity_s_in = d->velocity_s_in[0];
+    p->velocity_s_out = d->velocity_s_out[0];
+    p->uy_in = d->uy_in;
+    p->uy_out = d->uy_out;
+    for (int s = 0; s < 2; s++) {
+      p->pressure_n_in[s] = d->pressure_n_in[s];
+      p->pressure_n_out[s] = d->pressure_n_out[s];
+      p->pressure_s_in[s] = d->pressure_s_in[s];
+      p->pressure_s_out[s] = d->pressure_s_out[s];
+    }
+    if ((p->velocity_n_in || p->velocity_n_out) && (p->pressure_n_in[0] || p->pressure_n_out[0]))
+      return lbm_fail("the north row has both a pressure and a velocity condition");
+    if ((p->velocity_s_in || p->velocity_s_out) && (p->pressure_s_in[0] || p->pressure_s_out[0]))
+      return lbm_fail("the south row has both a pressure and a velocity condition");
+  } else {
+    for (int s = 0; s < 2; s++) {
+      p->pressure_n_in[s] = p->pressure_n_out[s] = p->pressure_s_in[s] = p->pressure_s_out[s] = 0;
+    }
+    p->zeroconcgrad_n = p->zeroconcgrad_s = 0;
+    // z faces: [0] bottom (z = 0), [1] top (z = LZ-1)
+    const int pin[2] = {d->pressure_b_in[0], d->pressure_t_in[0]};
+    const int pout[2] = {d->pressure_b_out[0], d->pressure_t_out[0]};
+    const int vin[2] = {d->velocity_b_in[0], d->velocity_t_in[0]};
+    const int vout[2] = {d->velocity_b_out[0], d->velocity_t_out[0]};
+    const int cin[2] = {d->constcon_b_in, d->constcon_t_in};
+    const int cout[2] = {d->constcon_b_out, d->constcon_t_out};
+    const int zg[2] = {d->zeroconcgrad_b, d->zeroconcgrad_t};
+    static const char *face[2] = {"bottom (b)", "top (t)"};
+    for (int f = 0; f < 2; f++) {
+      if ((pin[f] != 0) + (pout[f] != 0) + (vin[f] != 0) + (vout[f] != 0) > 1)
+        return lbm_fail("the %s face has more than one flow condition", face[f]);
+      if ((cin[f] != 0) + (cout[f] != 0) + (zg[f] != 0) > 1)
+        return lbm_fail("the %s face has more than one solute condition", face[f]);
+      const double rho_b = pin[f] ? d->rho_in : d->rho_out;
+      if (pin[f] || pout[f]) {
+        p->zface_flow[f] = LBM_ZFACE_PRESSURE;
+        if (f) p->rho_top = rho_b;
+        else p->rho_bot = rho_b;
+      } else if (vin[f] || vout[f]) {
+        p->zface_flow[f] = LBM_ZFACE_VELOCITY;
+        p->zface_uz[f] = vin[f] ? d->uz_in : d->uz_out;
+      } else {
+        p->zface_flow[f] = LBM_ZFACE_NONE;
+      }
+      if (nsubs == 2 && (cin[f] || cout[f])) {
+        p->zface_conc[f] = LBM_ZCONC_DIRICHLET;
+        const double c = cin[f] ? d->rho_sigma_in : d->rho_sigma_out;
+        if (f) p->c_top = c;
+        else p->c_bot = c;
+      } else if (nsubs == 2 && zg[f]) {
+        p->zface_conc[f] = LBM_ZCONC_ZEROGRAD;
+      } else {
+        p->zface_conc[f] = LBM_ZCONC_NONE;
+      }
+    }
+  }
   return 0;
 }
 

@@ -69,6 +69,11 @@ struct orc {
   int time;
   double flux, fmass;
   double *ns;           /* [N] Walsh partial bounce-back fraction, or NULL */
+  /* z-face conditions per face ([0] z = 0, [1] z = nk-1), resolved:
+   * flow 0 none, 1 pressure, 2 velocity; solute 0 none, 1 Dirichlet,
+   * 2 zero gradient; zfval: rho_b or rho_A u_z; zcval: Dirichlet c */
+  int zflow[2], zconc[2];
+  double zfval[2], zcval[2];
 };
 
 static int g_threads = 0;
@@ -137,6 +142,14 @@ orc *orc_create(const orc_params *p) {
     for (; a < 19; a++) o->wt[a] = W2;
   }
   for (a = 0; a < p->nsubs; a++) o->inv_tau[a] = 1. / p->tau[a];
+  for (a = 0; a < 2; a++) {
+    int fk = p->zface_flow[a], ck = p->zface_conc[a];
+    o->zflow[a] = fk == 2 ? 1 : fk == 3 ? 2 : fk == 1 ? 0 : (p->zface_pressure ? 1 : 0);
+    o->zfval[a] = o->zflow[a] == 2 ? p->rho_A[0] * p->zface_uz[a] : (a ? p->rho_top : p->rho_bot);
+    o->zconc[a] = p->nsubs != 2 ? 0 : ck == 2 ? 1 : ck == 3 ? 2 : ck == 1 ? 0 : (p->zface_solute ? (a ? 2 : 1) : 0);
+    o->zcval[a] = a ? p->c_top : p->c_bot;
+    if (p->dims != 3) o->zflow[a] = o->zconc[a] = 0;
+  }
   o->time = 0;
   return o;
 }
@@ -428,8 +441,43 @@ static double bc_rho(const orc *o, int s, int which, int t) {
   return rho;
 }
 
-/* k_sysbound_pressure_{n,s}_{1,2}: src/system_boundary_kernels.c:4-451 */
-static void bc2d_pressure(orc *o, int s, int north, int phase, double rho_b) {
+/* u_y of a velocity row (which: 0 n_in, 1 n_out, 2 s_in, 3 s_out) with the
+ * pressure rows' ramp, from 0 instead of from rho_A (no reference code: the
+ * reference parses the velocity_{n,s} flags and never applies them) */
+static double bc_uy(const orc *o, int which, int t) {
+  const orc_params *p = &o->p;
+  double u = (which == 0 || which == 2) ? p->uy_in : p->uy_out, ramp;
+  if (p->bc_ramp_start >= 0 && p->bc_ramp_stop >= 0) {
+    if (t < p->bc_ramp_start) {
+      u = 0.0;
+    } else if (t < p->bc_ramp_stop) {
+      ramp = ((double)(t - p->bc_ramp_start)) / ((double)(p->bc_ramp_stop - p->bc_ramp_start));
+      u *= ramp;
+    }
+  }
+  return u;
+}
+
+/* phase-1 store of a velocity row's unknown a at node (i, j): slot a at
+ * the pull source x - c_a, or the own slot opp(a) when that source is solid
+ * (the unknown replaces the bounced population; the 3-D faces' rule) */
+static void row_write_vel(orc *o, int i, int j, long n, int a, double v) {
+  int oa = (a & 1) ? a + 1 : a - 1;
+  long src = nb_index(o, i - VX[a], j - VY[a], 0);
+  if (!is_solid(o, src))
+    fslot(o, 0, a)[src] = v;
+  else
+    fslot(o, 0, oa)[n] = v;
+}
+
+/* k_sysbound_pressure_{n,s}_{1,2}: src/system_boundary_kernels.c:4-451.
+ * velocity: the velocity row instead, x = rho_b = rho_A u_y being the row's
+ * prescribed y momentum; with x the node's y momentum in both forms
+ * (north: x = P + 2M - (rho_b - rho_A) from the mass balance), the three
+ * unknowns follow from x the same way.  The reference has no velocity row,
+ * so its phase-1 placement rule (set_f1d_d: own slot a when the source is
+ * solid) is not followed there: row_write_vel. */
+static void bc2d_pressure(orc *o, int s, int north, int phase, double rho_b, int velocity) {
   const orc_params *p = &o->p;
   int j = north ? p->nj - 1 : 0;
   for (int i = 0; i < p->ni; i++) {
@@ -444,10 +492,15 @@ static void bc2d_pressure(orc *o, int s, int north, int phase, double rho_b) {
       x = f[C_] + f[E_] + f[W_] + f[N_] + f[N_] + f[NE_] + f[NE_] + f[NW_] + f[NW_];
       x -= rho_b;
       x += p->rho_A[s];
+      if (velocity) x = rho_b;
       f[S_] = f[N_] - (2. / 3.) * x;
       f[SW_] = f[NE_] + (1. / 2.) * (f[E_] - f[W_]) - (1. / 6.) * x;
       f[SE_] = f[NW_] + (1. / 2.) * (f[W_] - f[E_]) - (1. / 6.) * x;
-      if (phase == 1) {
+      if (phase == 1 && velocity) {
+        row_write_vel(o, i, j, n, S_, f[S_]);
+        row_write_vel(o, i, j, n, SE_, f[SE_]);
+        row_write_vel(o, i, j, n, SW_, f[SW_]);
+      } else if (phase == 1) {
         set_f(o, s, i, j, 0, n, -VX[S_], -VY[S_], 0, S_, 0, f[S_]);
         set_f(o, s, i, j, 0, n, -VX[SE_], -VY[SE_], 0, SE_, 0, f[SE_]);
         set_f(o, s, i, j, 0, n, -VX[SW_], -VY[SW_], 0, SW_, 0, f[SW_]);
@@ -460,10 +513,15 @@ static void bc2d_pressure(orc *o, int s, int north, int phase, double rho_b) {
       x = -f[C_] - f[E_] - f[W_] - f[S_] - f[S_] - f[SE_] - f[SE_] - f[SW_] - f[SW_];
       x += rho_b;
       x -= p->rho_A[s];
+      if (velocity) x = rho_b;
       f[N_] = f[S_] + (2. / 3.) * x;
       f[NE_] = f[SW_] + (1. / 2.) * (f[W_] - f[E_]) + (1. / 6.) * x;
       f[NW_] = f[SE_] + (1. / 2.) * (f[E_] - f[W_]) + (1. / 6.) * x;
-      if (phase == 1) {
+      if (phase == 1 && velocity) {
+        row_write_vel(o, i, j, n, N_, f[N_]);
+        row_write_vel(o, i, j, n, NW_, f[NW_]);
+        row_write_vel(o, i, j, n, NE_, f[NE_]);
+      } else if (phase == 1) {
         set_f(o, s, i, j, 0, n, -VX[N_], -VY[N_], 0, N_, 0, f[N_]);
         set_f(o, s, i, j, 0, n, -VX[NW_], -VY[NW_], 0, NW_, 0, f[NW_]);
         set_f(o, s, i, j, 0, n, -VX[NE_], -VY[NE_], 0, NE_, 0, f[NE_]);
@@ -537,12 +595,20 @@ static void bc2d_zerograd(orc *o, int north, int phase) {
  *     f_BS = f_TN - J/6 + Ny,  f_BN = f_TS - J/6 - Ny
  *   Nx = ((f_E + f_NE + f_SE) - (f_W + f_NW + f_SW)) / 2, Ny likewise in y.
  * The shift w_a rho_A cancels in every line because the weights of each
- * pair are equal and the weights of P + 2M sum to one.
+ * pair are equal and the weights of P + 2M sum to one.  J is the node's z
+ * momentum sum_a c_a,z f~_a after the condition.
  *
- * Solute: Dirichlet concentration c_b at z=0 -- the five unknown
- * populations are w_a c*, c* = (c_b - sum of the 14 known) / (sum of their
- * weights = 1/6); zero gradient at z=nk-1 -- unknown w_a c*, c* = (sum of
- * the five outgoing) / (1/6), the 3-D form of k_sysbound_zeroconcgrad_n.
+ * Flow, Zou-He velocity (the reference's CPU path has the compressible
+ * form, rho = (P + 2M)/(1 + u) with c = rho u, for the top inflow and the
+ * bottom outflow, src/bcs.c:766-862): the same five lines with J the
+ * prescribed z momentum rho_A u_z -- in the incompressible model the
+ * velocity is j / rho_A -- and the density following from the unknowns.
+ *
+ * Solute, per face: Dirichlet concentration (c_bot or the map at z=0, c_top
+ * at z=nk-1) -- the five unknown populations are w_a c*, c* = (c_b - sum of
+ * the 14 known) / (sum of their weights = 1/6); zero gradient -- unknown
+ * w_a c*, c* = (sum of the five outgoing) / (1/6), the 3-D form of
+ * k_sysbound_zeroconcgrad_n.
  *
  * Phase 1 (before the odd half-step) writes each unknown where the pull
  * will read it: slot a at x - c_a, or the own slot opp(a) when x - c_a is
@@ -566,7 +632,6 @@ static void zface_write(orc *o, int s, int phase, int i, int j, int k, long n, i
 
 static void zface_bcs(orc *o, int phase, int t) {
   const orc_params *p = &o->p;
-  int solute_on = p->nsubs == 2 && p->zface_solute;
   (void)t;
   for (int top = 0; top < 2; top++) {
     int k = top ? p->nk - 1 : 0;
@@ -575,7 +640,7 @@ static void zface_bcs(orc *o, int phase, int t) {
         long n = (long)i + (long)p->ni * j + (long)p->ni * p->nj * k;
         double f[19], g[19];
         if (is_solid(o, n)) continue;
-        if (p->zface_pressure) {
+        if (o->zflow[top]) {
           double P, M = 0., J, Nx, Ny, v[5];
           const int *kn = top ? UP5 : DN5;
           if (phase == 1)
@@ -588,7 +653,7 @@ static void zface_bcs(orc *o, int phase, int t) {
           Nx = ((f[1] + f[5] + f[8]) - (f[2] + f[7] + f[6])) / 2.;
           Ny = ((f[3] + f[5] + f[7]) - (f[4] + f[6] + f[8])) / 2.;
           if (!top) {
-            J = (p->rho_bot - p->rho_A[0]) - (P + 2. * M);
+            J = o->zflow[0] == 2 ? o->zfval[0] : (o->zfval[0] - p->rho_A[0]) - (P + 2. * M);
             v[0] = f[10] + J / 3.;
             v[1] = f[12] + J / 6. - Nx;
             v[2] = f[14] + J / 6. + Nx;
@@ -596,7 +661,7 @@ static void zface_bcs(orc *o, int phase, int t) {
             v[4] = f[18] + J / 6. + Ny;
             for (int q = 0; q < 5; q++) zface_write(o, 0, phase, i, j, k, n, UP5[q], v[q]);
           } else {
-            J = (P + 2. * M) - (p->rho_top - p->rho_A[0]);
+            J = o->zflow[1] == 2 ? o->zfval[1] : (P + 2. * M) - (o->zfval[1] - p->rho_A[0]);
             v[0] = f[9] - J / 3.;
             v[1] = f[11] - J / 6. + Nx;
             v[2] = f[13] - J / 6. - Nx;
@@ -605,21 +670,23 @@ static void zface_bcs(orc *o, int phase, int t) {
             for (int q = 0; q < 5; q++) zface_write(o, 0, phase, i, j, k, n, DN5[q], v[q]);
           }
         }
-        if (solute_on) {
+        if (o->zconc[top]) {
+          const int dir = o->zconc[top] == 1;
           double cs, known = 0.;
           if (phase == 1)
             gather_pull(o, 1, i, j, k, n, g);
           else
             gather_local(o, 1, n, g);
-          if (!top) {
-            double cb = p->c_bot_map ? p->c_bot_map[(long)i + (long)p->ni * j] : p->c_bot;
+          if (dir)
             for (int a = 0; a < 9; a++) known += g[a];
+          if (!top) {
+            double cb = p->c_bot_map ? p->c_bot_map[(long)i + (long)p->ni * j] : o->zcval[0];
             for (int q = 0; q < 5; q++) known += g[DN5[q]];
-            cs = (cb - known) / (1. / 6.);
+            cs = dir ? (cb - known) / (1. / 6.) : known / (1. / 6.);
             for (int q = 0; q < 5; q++) zface_write(o, 1, phase, i, j, k, n, UP5[q], o->wt[UP5[q]] * cs);
           } else {
             for (int q = 0; q < 5; q++) known += g[UP5[q]];
-            cs = known / (1. / 6.);
+            cs = dir ? (o->zcval[1] - known) / (1. / 6.) : known / (1. / 6.);
             for (int q = 0; q < 5; q++) zface_write(o, 1, phase, i, j, k, n, DN5[q], o->wt[DN5[q]] * cs);
           }
         }
@@ -633,14 +700,20 @@ void orc_bcs(orc *o, int phase, int t) {
   const orc_params *p = &o->p;
   if (o->D == 2) {
     for (int s = 0; s < p->nsubs; s++) {
-      if (p->pressure_n_in[s]) bc2d_pressure(o, s, 1, phase, bc_rho(o, s, 0, t));
-      if (p->pressure_n_out[s]) bc2d_pressure(o, s, 1, phase, bc_rho(o, s, 1, t));
-      if (p->pressure_s_in[s]) bc2d_pressure(o, s, 0, phase, bc_rho(o, s, 2, t));
-      if (p->pressure_s_out[s]) bc2d_pressure(o, s, 0, phase, bc_rho(o, s, 3, t));
+      if (p->pressure_n_in[s]) bc2d_pressure(o, s, 1, phase, bc_rho(o, s, 0, t), 0);
+      if (p->pressure_n_out[s]) bc2d_pressure(o, s, 1, phase, bc_rho(o, s, 1, t), 0);
+      if (p->pressure_s_in[s]) bc2d_pressure(o, s, 0, phase, bc_rho(o, s, 2, t), 0);
+      if (p->pressure_s_out[s]) bc2d_pressure(o, s, 0, phase, bc_rho(o, s, 3, t), 0);
+      if (s == 0) {
+        if (p->velocity_n_in) bc2d_pressure(o, 0, 1, phase, p->rho_A[0] * bc_uy(o, 0, t), 1);
+        if (p->velocity_n_out) bc2d_pressure(o, 0, 1, phase, p->rho_A[0] * bc_uy(o, 1, t), 1);
+        if (p->velocity_s_in) bc2d_pressure(o, 0, 0, phase, p->rho_A[0] * bc_uy(o, 2, t), 1);
+        if (p->velocity_s_out) bc2d_pressure(o, 0, 0, phase, p->rho_A[0] * bc_uy(o, 3, t), 1);
+      }
       if (s == 1 && p->zeroconcgrad_n) bc2d_zerograd(o, 1, phase);
       if (s == 1 && p->zeroconcgrad_s) bc2d_zerograd(o, 0, phase);
     }
-  } else if (p->zface_pressure || (p->zface_solute && p->nsubs == 2)) {
+  } else if (o->zflow[0] || o->zflow[1] || o->zconc[0] || o->zconc[1]) {
     zface_bcs(o, phase, t);
   }
 }

@@ -18,10 +18,11 @@
  *   BC host logic   bcs_1 / bcs_2             src/bcs.c:226-557, 1179-1510
  *   halo pack/unpack                          src/boundary_kernels.c:1-139
  *   host reductions compute_{max,min,ave}_*   src/compute.c:1159-1416
- * plus the parts that have no reference code (3-D z-face Zou-He pressure,
- * solute Dirichlet inlet / zero-gradient outlet on z faces, FP64 mass /
- * velocity / plane-flux reductions), whose definitions are stated in
- * lbm_oracle.c next to the code.
+ * plus the parts that have no reference GPU code (3-D z-face Zou-He
+ * pressure and velocity, whose top/bottom forms the reference's CPU path has
+ * in src/bcs.c:592-862; solute Dirichlet / zero-gradient z faces; 2-D Zou-He
+ * velocity rows; FP64 mass / velocity / plane-flux reductions), whose
+ * definitions are stated in lbm_oracle.c next to the code.
  *
  * Which check pins which function is listed in the header of lbm_oracle.c.
  */
@@ -68,6 +69,21 @@ typedef struct orc_params {
   /* plane (index along the last axis) whose solute moments are summed
    * during each step; -1 = none */
   int flux_plane;
+
+  /* Zou-He velocity rows (2-D, flow): the row's y momentum rho_A u_y is
+   * prescribed, u_y = uy_in for *_in and uy_out for *_out, ramped from 0 by
+   * bc_ramp_start/stop like the pressure rows' density */
+  int velocity_n_in, velocity_n_out, velocity_s_in, velocity_s_out;
+  double uy_in, uy_out;
+  /* per-face 3-D conditions, [0] z = 0, [1] z = nk-1; 0 keeps what
+   * zface_pressure / zface_solute select for the face.  Flow: 1 none,
+   * 2 pressure (rho_bot / rho_top), 3 velocity (zface_uz[face]: the face's
+   * z momentum rho_A u_z is prescribed).  Solute: 1 none, 2 Dirichlet
+   * (c_bot and c_bot_map at z = 0, c_top at z = nk-1), 3 zero gradient. */
+  int zface_flow[2];
+  double zface_uz[2];
+  int zface_conc[2];
+  double c_top;
 } orc_params;
 
 typedef struct orc orc;

@@ -39,6 +39,11 @@ class OrcParams(ctypes.Structure):
         ("zface_solute", ctypes.c_int), ("c_bot", ctypes.c_double),
         ("c_bot_map", ctypes.POINTER(ctypes.c_double)),
         ("flux_plane", ctypes.c_int),
+        ("velocity_n_in", ctypes.c_int), ("velocity_n_out", ctypes.c_int),
+        ("velocity_s_in", ctypes.c_int), ("velocity_s_out", ctypes.c_int),
+        ("uy_in", ctypes.c_double), ("uy_out", ctypes.c_double),
+        ("zface_flow", ctypes.c_int * 2), ("zface_uz", ctypes.c_double * 2),
+        ("zface_conc", ctypes.c_int * 2), ("c_top", ctypes.c_double),
     ]
 
 
@@ -136,6 +141,14 @@ def make_params(**kw):
     p.zface_solute = kw.get("zface_solute", 0)
     p.c_bot = kw.get("c_bot", 1.0)
     p.flux_plane = kw.get("flux_plane", -1)
+    for name in ("velocity_n_in", "velocity_n_out", "velocity_s_in", "velocity_s_out"):
+        setattr(p, name, kw.get(name, 0))
+    p.uy_in, p.uy_out = kw.get("uy_in", 0.0), kw.get("uy_out", 0.0)
+    for name in ("zface_flow", "zface_uz", "zface_conc"):
+        v = kw.get(name, (0, 0))
+        arr = getattr(p, name)
+        arr[0], arr[1] = v[0], v[1]
+    p.c_top = kw.get("c_top", 0.0)
     return p
 
 

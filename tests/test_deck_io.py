@@ -90,6 +90,79 @@ def test_deck_params_mapping(io):
     assert p.flux_plane == -1 and p.storage == 0 and p.compute == 0
 
 
+def labelled_deck(tmp_path, edits, name="params.in"):
+    """the reference deck with the values of some labels replaced"""
+    lines = []
+    for lab, val in deck_lines():
+        lines.append(f"{lab} {edits.pop(lab, val)}")
+    assert not edits, edits
+    path = tmp_path / name
+    path.write_text("\n".join(lines) + "\n")
+    return str(path)
+
+
+# the reference deck's own 2-D boundary flags, cleared for 3-D decks
+NO_2D = {"pressure_n_in[0]": 0, "pressure_s_in[0]": 0, "zeroconcgrad_n": 0, "zeroconcgrad_s": 0}
+
+
+def test_deck_params_3d_faces(io, tmp_path):
+    """3-D decks: the t/b pressure and velocity flags become z-face
+    conditions (t = z = LZ-1, b = z = 0) with rho_in/rho_out and
+    uz_in/uz_out; constcon_{t,b} a Dirichlet face with rho_sigma_in/out,
+    zeroconcgrad_{t,b} a zero-gradient face"""
+    edits = dict(NO_2D, **{"pressure_b_out[0]": 1, "velocity_t_in[0]": 1, "constcon_t_in": 1,
+                           "zeroconcgrad_b": 1, "rho_sigma_in": "0.75"})
+    d = io.read_deck(labelled_deck(tmp_path, edits), dims=3, nsubs=2)
+    p = io.deck_params(d, dims=3, nsubs=2)
+    assert tuple(p.zface_flow) == (2, 3)  # bottom pressure, top velocity
+    assert p.rho_bot == 0.99999 and tuple(p.zface_uz) == (0.0, -0.01)
+    assert tuple(p.zface_conc) == (3, 2) and p.c_top == 0.75
+    assert p.zface_pressure == 0 and p.zface_solute == 0
+    assert tuple(p.pressure_n_in) == (0, 0) and p.zeroconcgrad_n == 0
+    # the other combination: velocity out at the bottom, pressure in at the top
+    edits = dict(NO_2D, **{"velocity_b_out[0]": 1, "pressure_t_in[0]": 1, "constcon_b_in": 1})
+    d = io.read_deck(labelled_deck(tmp_path, edits), dims=3, nsubs=2)
+    p = io.deck_params(d, dims=3, nsubs=2)
+    assert tuple(p.zface_flow) == (3, 2) and p.zface_uz[0] == -0.01 and p.rho_top == 1.00001
+    assert tuple(p.zface_conc) == (2, 1) and p.c_bot == 0.0  # rho_sigma_in is 0 in the deck
+    # no flags: both faces explicitly without a condition (periodic)
+    d = io.read_deck(labelled_deck(tmp_path, dict(NO_2D)), dims=3, nsubs=1)
+    p = io.deck_params(d, dims=3, nsubs=1)
+    assert tuple(p.zface_flow) == (1, 1) and tuple(p.zface_conc) == (1, 1)
+
+
+def test_deck_params_2d_velocity_rows(io, tmp_path):
+    edits = {"pressure_s_in[0]": 0, "velocity_s_in[0]": 1, "uy_in": "0.002"}
+    d = io.read_deck(labelled_deck(tmp_path, edits), dims=2, nsubs=2)
+    p = io.deck_params(d, dims=2, nsubs=2)
+    assert p.velocity_s_in == 1 and p.uy_in == 0.002 and tuple(p.pressure_n_in) == (2, 0)
+    assert tuple(p.pressure_s_in) == (0, 0)
+
+
+@pytest.mark.parametrize("dims,edits,match", [
+    (3, {}, r"pressure_n_in\[0\] = 2 is set"),  # the reference deck's 2-D rows in 3-D
+    (2, {"zeroconcgrad_full": 1}, "zeroconcgrad_full = 1 is set"),
+    (2, {"pressure_e_in[0]": 1}, r"pressure_e_in\[0\] = 1 is set"),
+    (2, {"velocity_w_out[1]": 1}, r"velocity_w_out\[1\] = 1 is set"),
+    (2, {"velocity_n_in[1]": 1}, r"velocity_n_in\[1\] = 1 is set"),
+    (2, {"pressure_t_in[0]": 1}, r"pressure_t_in\[0\] = 1 is set"),
+    (2, {"constflx_s_in": 1}, "constflx_s_in = 1 is set"),
+    (2, {"GZL": 1}, "GZL = 1 is set"),
+    (2, {"bc_slip_north": 1}, "bc_slip_north = 1 is set"),
+    (3, dict(NO_2D, **{"pressure_t_in[1]": 1}), r"pressure_t_in\[1\] = 1 is set"),
+    (3, dict(NO_2D, **{"zeroconcgrad_s": 0, "constcon_n_in": 1}), "constcon_n_in = 1 is set"),
+    (3, dict(NO_2D, **{"pressure_t_in[0]": 1, "velocity_t_out[0]": 1}), "top .t. face has more than one flow"),
+    (3, dict(NO_2D, **{"constcon_b_in": 1, "zeroconcgrad_b": 1}), "bottom .b. face has more than one solute"),
+    (2, {"velocity_n_in[0]": 1}, "north row has both a pressure and a velocity condition"),
+])
+def test_deck_params_rejects_flags_without_condition(io, tmp_path, dims, edits, match):
+    """a set boundary flag this build cannot apply is an error naming it,
+    not a silent no-op as in the reference's GPU build"""
+    d = io.read_deck(labelled_deck(tmp_path, dict(edits)), dims=dims, nsubs=2)
+    with pytest.raises(Exception, match=match):
+        io.deck_params(d, dims=dims, nsubs=2)
+
+
 def edited_deck(tmp_path, edits):
     lines = [f"{lab} {val}" for lab, val in deck_lines()]
     for k, v in edits.items():

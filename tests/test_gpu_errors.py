@@ -27,6 +27,18 @@ def test_calls_before_and_with_bad_solids():
     g.init_rest()
     g.run(2)
     assert np.all(np.isfinite(g.mv(0)[0]))
+    # a rejected lbm_set_solids releases the previous lattice and leaves
+    # none behind: steps fail cleanly until a later call succeeds
+    with pytest.raises(L.LbmError, match="no fluid node"):
+        g.set_solids(np.ones((7, 9), np.uint8))
+    with pytest.raises(L.LbmError, match="lbm_set_solids has not been called"):
+        g.run(1)
+    with pytest.raises(L.LbmError, match="lbm_set_solids has not been called"):
+        g.mv(0)
+    g.set_solids(np.zeros((7, 9), np.uint8))
+    g.init_rest()
+    g.run(2)
+    assert g.reduce_sums()[4] == 63
 
 
 def test_bad_subs_nodes_and_modes():

@@ -439,12 +439,16 @@ def test_2d_reference_bcs_fp32_storage(compute):
 
 
 @pytest.mark.parametrize("case", ["2d_pressure_rows_solute", "3d_zface_solute_walsh", "2d_ramp_not_captured",
-                                  "3d_zface_solute_fp32", "3d_zface_solute_fp16"])
+                                  "3d_zface_solute_fp32", "3d_zface_solute_fp16", "3d_solute_window"])
 def test_captured_step_chunks_equal_stepwise_launches(case):
     """lbm_run launches its steps in captured HIP graphs of 64 steps where
     the launches repeat; three runs (the graphs replayed several times, with
     eager steps, a parity change and synchronisations between them) equal
-    the same steps launched one by one, bit for bit, after every run"""
+    the same steps launched one by one, bit for bit, after every run.
+    3d_solute_window: the solute is active in steps 150..360 only; runs
+    wholly before, wholly inside and wholly after that window replay graphs
+    captured on their own side of it (a graph captured before the window
+    must not be replayed inside it, nor one captured inside after it)."""
     say(f"case captured chunks {case}")
     L = _pkg.load()
     rng = np.random.default_rng(515)
@@ -471,6 +475,9 @@ def test_captured_step_chunks_equal_stepwise_launches(case):
         elif case.endswith("fp16"):
             kw.update(storage=2, compute=1, store_shift_c=0.5)
             ns = None
+        elif case == "3d_solute_window":
+            kw.update(sigma_t_on=150, sigma_t_off=360)
+            ns = None
     f, gg = random_state(rng, Q, N, 2)
     lats = []
     for _ in range(2):
@@ -481,7 +488,8 @@ def test_captured_step_chunks_equal_stepwise_launches(case):
         lat.upload_f(1, gg)
         lats.append(lat)
     a, b = lats
-    for n in (150, 1, 200, 1, 129):
+    runs = (140, 10, 150, 60, 150) if case == "3d_solute_window" else (150, 1, 200, 1, 129)
+    for n in runs:
         if n == 1:
             a.step()
         else:
@@ -493,6 +501,6 @@ def test_captured_step_chunks_equal_stepwise_launches(case):
         if n > 1:
             for s in range(2):
                 assert all(np.array_equal(x, y) for x, y in zip(a.mv(s), b.mv(s))), (n, s)
-    say(f"{case}: equal after 481 steps")
+    say(f"{case}: equal after {sum(runs)} steps")
     a.close()
     b.close()

@@ -141,6 +141,7 @@ def test_benchmark_geometries_recorded(name, args, sha, phi):
     (dict(storage=3), "bad storage/compute"),
     (dict(storage=2, compute=0), "FP16 storage runs the FP32 collision"),
     (dict(dims=3, ni=2048, nj=1024, nk=1024), "too large for 32-bit node indices"),
+    (dict(ni=40000, nj=2), "LX = 40000 above 32767 is not supported"),
     (dict(dims=3, nk=5, flux_plane=5), "flux_plane 5 lies outside"),
     (dict(flux_plane=6), "flux_plane 6 lies outside"),
 ])
