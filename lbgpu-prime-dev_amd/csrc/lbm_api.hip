@@ -426,8 +426,9 @@ int upload_band_map(lbm_handle *h, int bs, long nb, long per, int **dst) {
   return 0;
 }
 
-// The 3-D lattice's block maps.  Odd half-step (kBlockOdd): only when an
-// XCD's band of one plane holds at least 3 blocks.  Measured on sphere packs
+// The 3-D lattice's block maps.  Odd half-step (blocks of odd_block(): 512
+// nodes, 256 with FP16 storage): only when an XCD's band of one plane holds
+// at least 3 blocks.  Measured on sphere packs
 // of porosity 0.40, odd + even step: 256^3 (6.7 blocks) flow FP64 0.777 ->
 // 0.727 ms, flow + solute FP64 1.591 -> 1.497 ms, FP32 0.785 -> 0.741 ms;
 // 192^3 (3.9 blocks) 1 to 3 % faster; 128^3 (1.7 blocks) 2 to 4 % slower,
@@ -435,10 +436,12 @@ int upload_band_map(lbm_handle *h, int bs, long nb, long per, int **dst) {
 // step (kBlock, all 8 * nblk8 blocks so that every gradient slot is
 // written): at 128^3 0.319 -> 0.305 s of adjoint kernels per sweep.
 int build_band_maps(lbm_handle *h) {
-  const long nbo = blocks_for(h->nf, kBlockOdd);
-  if ((double)h->nf / h->p.nk / 8.0 >= 3.0 * kBlockOdd) {
+  // the odd step's node block for this storage (FP16: 256 nodes)
+  const int bso = odd_block((int)h->st_bytes);
+  const long nbo = blocks_for(h->nf, bso);
+  if ((double)h->nf / h->p.nk / 8.0 >= 3.0 * bso) {
     const long per = (nbo + 7) / 8;
-    if (int rc = upload_band_map(h, kBlockOdd, nbo, per, &h->bmap)) return rc;
+    if (int rc = upload_band_map(h, bso, nbo, per, &h->bmap)) return rc;
     h->bmap8 = (int)per;
   }
   return upload_band_map(h, kBlock, 8L * h->nblk8, h->nblk8, &h->amap);
@@ -450,7 +453,7 @@ template <typename ST, typename CT>
 dim3 odd_grid(const lbm_handle *h, StepArgs<ST, CT> &A, int bs) {
   A.nblk8 = (int)((blocks_for(h->nf, bs) + 7) / 8);
   A.bmap = nullptr;
-  if (bs == kBlockOdd && h->bmap) {
+  if (bs == odd_block((int)h->st_bytes) && h->bmap) {
     A.bmap = h->bmap;
     A.nblk8 = h->bmap8;
   }

@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 #include "lbm_index.hpp"
 
 namespace lbm {
@@ -29,13 +31,17 @@ constexpr int kBlock = 256;
 // keep more requests in flight per CU with 8-wave workgroups: at 256^3
 // (D3Q19, FP64, porosity 0.40) the odd step takes 0.437 ms at 512 threads
 // against 0.457 ms at 256 and 0.491 ms at 1024 (contiguous block order).
-// With FP16 storage (flow + solute) it stays at 256 threads in the
-// contiguous order: 512 threads cost 4 % there (0.269 -> 0.280 ms), and in
-// the row-band order the step pair gained 4 % at 256^3 but lost 4 % at
-// 192^3 (porosity 0.45).  The even step is best at 256.
+// With FP16 storage (flow + solute) it stays at 256 threads: in the
+// row-band order with 256-node blocks its odd step took 0.242 against
+// 0.267 ms at 256^3 and 0.119 against 0.124 ms at 192^3 (porosity 0.45)
+// compared with the contiguous order; 512 threads in the row-band order
+// gained 4 % at 256^3 but lost 4 % at 192^3.  The even step is best at 256.
 constexpr int kBlockOdd = 512;
 template <bool ODD, typename ST>
 constexpr int half_step_block() { return ODD && sizeof(ST) >= 4 ? kBlockOdd : kBlock; }
+// the odd half-step's node block for a storage type (the row-band block map
+// of a lattice is built for it)
+inline int odd_block(int storage_bytes) { return storage_bytes >= 4 ? kBlockOdd : kBlock; }
 
 // D3Q19 velocity set in the reference's order (src/lbgpu_prime.h:78-84):
 // opposite directions adjacent, D2Q9 first.
@@ -242,6 +248,26 @@ __device__ __forceinline__ void scatter2(const StepArgs<ST, CT> &A, int idx, con
   }
 }
 
+// acc + c x for a compile-time velocity component c in {-1, 0, 1}.  The
+// product by +-1 is exact and adding an exact zero leaves acc unchanged, so
+// for finite x the value equals the written acc + (CT)c * x, without the
+// multiplications by zero that the compiler must otherwise keep (NaN and
+// signed-zero rules).
+template <int C, typename CT>
+__device__ __forceinline__ CT add_c(CT acc, CT x) {
+  if constexpr (C == 0) return acc;
+  else if constexpr (C > 0) return acc + x;
+  else return acc - x;
+}
+// c . u over the nonzero components of c, in the written order x, y, z
+template <int CXa, int CYa, int CZa, typename CT>
+__device__ __forceinline__ CT cdotu(CT u0, CT u1, CT u2) {
+  if constexpr (CXa != 0) return add_c<CZa, CT>(add_c<CYa, CT>(CXa > 0 ? u0 : -u0, u1), u2);
+  else if constexpr (CYa != 0) return add_c<CZa, CT>(CYa > 0 ? u1 : -u1, u2);
+  else if constexpr (CZa != 0) return CZa > 0 ? u2 : -u2;
+  else return (CT)0;
+}
+
 // Moments, force, macroscopic write and BGK relaxation of one subs, in the
 // order of src/k_stream_collide_stream.c:80-390 and
 // src/lattice.c:2850-2944.  SUBS 0: He-Luo incompressible flow on shifted
@@ -262,11 +288,24 @@ __device__ __forceinline__ void collide(const StepArgs<ST, CT> &A, int idx, CT (
     u[0] = (CT)0;
     u[1] = (CT)0;
     u[2] = (CT)0;
+    if constexpr (sizeof(CT) == 4) {
+      // FP32 collision: the zero terms skipped (same values; 16 % fewer
+      // instructions: FP16 odd step 3 % faster)
+      [&]<int... A>(std::integer_sequence<int, A...>) {
+        ((u[0] = add_c<CX(A), CT>(u[0], f[A]), u[1] = add_c<CY(A), CT>(u[1], f[A]),
+          u[2] = D == 3 ? add_c<CZ(A), CT>(u[2], f[A]) : u[2]),
+         ...);
+      }(std::make_integer_sequence<int, Q>{});
+    } else {
+      // FP64: as written (the skipped form frees registers, and the flow +
+      // solute odd step's second workgroup per CU then slows it by 5 % at
+      // 256^3)
 #pragma unroll
-    for (int a = 0; a < Q; a++) {
-      u[0] += ((CT)CX(a)) * f[a];
-      u[1] += ((CT)CY(a)) * f[a];
-      if (D == 3) u[2] += ((CT)CZ(a)) * f[a];
+      for (int a = 0; a < Q; a++) {
+        u[0] += ((CT)CX(a)) * f[a];
+        u[1] += ((CT)CY(a)) * f[a];
+        if (D == 3) u[2] += ((CT)CZ(a)) * f[a];
+      }
     }
     if constexpr (sizeof(CT) == 4 && sizeof(ST) >= 4) {
 #pragma unroll
@@ -320,11 +359,22 @@ __device__ __forceinline__ void collide(const StepArgs<ST, CT> &A, int idx, CT (
     if (D == 3) usq += u[2] * u[2];
   }
   const CT it = SUBS == 0 ? A.inv_tau : A.inv_tau_s;
+  CT vd[Q];
+  if constexpr (sizeof(CT) == 4) {
+    [&]<int... A>(std::integer_sequence<int, A...>) {
+      ((vd[A] = cdotu<CX(A), CY(A), D == 3 ? CZ(A) : 0, CT>(u[0], u[1], u[2])), ...);
+    }(std::make_integer_sequence<int, Q>{});
+  } else {
+#pragma unroll
+    for (int a = 0; a < Q; a++) {
+      vd[a] = ((CT)CX(a)) * u[0] + ((CT)CY(a)) * u[1];
+      if (D == 3) vd[a] += ((CT)CZ(a)) * u[2];
+    }
+  }
 #pragma unroll
   for (int a = 0; a < Q; a++) {
     CT fa = f[a] * ((CT)1 - it);
-    CT vdotu = ((CT)CX(a)) * u[0] + ((CT)CY(a)) * u[1];
-    if (D == 3) vdotu += ((CT)CZ(a)) * u[2];
+    const CT vdotu = vd[a];
     const CT w = (CT)W<Q>(a);
     if (SUBS == 1) {
       fa += w * rho * ((CT)1 + (CT)3 * vdotu) * it;

@@ -210,6 +210,36 @@ def test_row_band_block_order(nsubs):
     g.close()
 
 
+def test_row_band_block_order_fp16():
+    """FP16 storage runs its odd half-step in 256-node blocks, with a
+    row-band map of its own (nf / nk / 8 >= 3 * 256 here): every node is
+    updated exactly once per half-step (a skipped or repeated block would be
+    off by far more than FP16 rounding), so 24 steps stay within the storage's
+    error of the FP64 oracle"""
+    say("case row-band block order fp16")
+    L = _pkg.load()
+    ni, nj, nk = 256, 160, 10
+    mask, _ = L.spheres3d(ni, nj, nk, 606, 3.0, 6.0, 0.45, zbuf=2)
+    nf = int((mask.reshape(-1) != 1).sum())
+    assert nf / nk / 8 >= 3 * 256, nf
+    kw = dict(dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(0.8, 0.75), zface_pressure=1, rho_bot=1.001,
+              rho_top=1.0, gaccel=(1e-6, -2e-6, 0, 0, 0, 0), zface_solute=1, c_bot=1.0, flux_plane=nk - 2)
+    o = Oracle(**kw)
+    o.set_solids(mask)
+    g = L.Lattice(mask=mask, storage=2, compute=1, **kw)
+    f, gg = random_state(np.random.default_rng(67), 19, ni * nj * nk, 2)
+    load_both(o, g, f, gg)
+    o.run(24)
+    g.run(24)
+    fl = mask.reshape(-1) != 1
+    for s in range(2):
+        for c in range(4):
+            r = relnorm(g.mv(s)[c][fl], o.mv(s)[c][fl])
+            say(f"fp16 band order subs {s} field {c}: {r:.2e}")
+            assert r <= 5e-3, (s, c, r)
+    g.close()
+
+
 @pytest.mark.parametrize("dims", [2, 3])
 def test_pest_partial_macro_write_and_upload(dims):
     """LBM_MACRO_PEST writes the flow velocity and the solute concentration
