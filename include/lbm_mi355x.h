@@ -143,6 +143,16 @@ int lbm_set_ns(lbm_handle *h, const double *ns);
  * nodes are not stored: download returns 0 there. */
 int lbm_upload_f(lbm_handle *h, int subs, const double *f1d);
 int lbm_download_f(lbm_handle *h, int subs, double *f1d);
+/* lbm_download_f without any allocation (no counterpart in the reference;
+ * for callers that cannot allocate device or host memory during a run, such
+ * as serving loops that checkpoint the state): the same [Q][N] values, staged
+ * in a static device array of the library instead of a buffer allocated per
+ * call.  Q x N may not exceed 8,388,608 (64 MiB of FP64: D3Q19 lattices up to
+ * 441,505 nodes, D2Q9 up to 932,067); larger lattices are rejected with an
+ * error.  Every handle on one GPU shares that GPU's array, and each call
+ * rewrites all of the Q x N elements it returns: calls that overlap on one
+ * GPU (handles used from several threads at once) race. */
+int lbm_download_f_noalloc(lbm_handle *h, int subs, double *f1d);
 
 /* init_problem for u = 0 (src/latman.c:1786-1893): flow f~ = 0, solute
  * g = w c0 (c0: [N] or NULL for 0) */

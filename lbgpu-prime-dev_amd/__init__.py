@@ -91,6 +91,7 @@ def lib():
             "lbm_set_solids": [vp, u8p],
             "lbm_upload_f": [vp, ip, dp],
             "lbm_download_f": [vp, ip, dp],
+            "lbm_download_f_noalloc": [vp, ip, dp],
             "lbm_init_rest": [vp, dp],
             "lbm_set_time": [vp, ip],
             "lbm_get_time": [vp, ctypes.POINTER(ctypes.c_int)],
@@ -255,6 +256,13 @@ class Lattice:
     def download_f(self, subs):
         a = np.empty((self.Q, self.N))
         _check(lib().lbm_download_f(self.h, subs, _dp(a)))
+        return a
+
+    def download_f_noalloc(self, subs):
+        """lbm_download_f_noalloc: as download_f, staged in the library's
+        static device array (no allocation)"""
+        a = np.empty((self.Q, self.N))
+        _check(lib().lbm_download_f_noalloc(self.h, subs, _dp(a)))
         return a
 
     def init_rest(self, c0=None):

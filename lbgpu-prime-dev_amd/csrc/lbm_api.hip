@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,19 @@
 using namespace lbm;
 
 static thread_local std::string g_err;
+
+// Staging memory of lbm_download_f_noalloc: one [Q][N] distribution in
+// FP64, at most kScratchDoubles elements (64 MiB: D3Q19 lattices up to
+// 441,505 nodes, D2Q9 up to 932,067).  Each GPU the library runs on holds
+// its own copy until the process ends.  In host code `scratch` only names
+// the symbol: its device address on a GPU comes from hipGetSymbolAddress
+// with that GPU current (scratch_on_device), and that address is what
+// kernels and copies take.
+constexpr long kScratchDoubles = 8L << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double *g_scratch[kMaxDevices];
+static std::mutex g_scratch_mu;
 
 extern "C" int lbm_fail(const char *fmt, ...) {
   char buf[1024];
@@ -98,6 +112,7 @@ struct lbm_handle {
   int flux_valid = 0;
   int time = 0;
   hipStream_t stream = nullptr;
+  double *scratch = nullptr;  // this device's address of the static staging array
   int profile = 0;
   std::vector<ProfPair> pending;
   std::vector<hipEvent_t> pool;
@@ -344,6 +359,23 @@ void free_lattice(lbm_handle *h) {
 int need_lattice(const lbm_handle *h) {
   if (!h) return lbm_fail("null handle");
   if (!h->F) return lbm_fail("lbm_set_solids has not been called");
+  return 0;
+}
+
+// the device address of the static staging array on the current device,
+// looked up once per device
+int scratch_on_device(double **out) {
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return lbm_fail("device %d beyond the library's %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *ptr = nullptr;
+    HIP_OK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return lbm_fail("no device address for the staging array");
+    g_scratch[dev] = (double *)ptr;
+  }
+  *out = g_scratch[dev];
   return 0;
 }
 
@@ -720,6 +752,11 @@ extern "C" int lbm_create(const lbm_params *p, lbm_handle **out) {
     delete h;
     return lbm_fail("hipStreamCreate failed");
   }
+  if (scratch_on_device(&h->scratch)) {
+    const std::string why = g_err;
+    lbm_destroy(h);
+    return lbm_fail("%s", why.c_str());
+  }
   if (p->c_bot_map && p->dims == 3) {
     size_t E = (size_t)h->E;
     if (hipMalloc(&h->cmap, E * sizeof(double)) != hipSuccess ||
@@ -922,6 +959,31 @@ extern "C" int lbm_upload_f(lbm_handle *h, int subs, const double *f1d) {
 }
 
 extern "C" int lbm_download_f(lbm_handle *h, int subs, double *f1d) { return transfer_f(h, subs, f1d, 0); }
+
+extern "C" int lbm_download_f_noalloc(lbm_handle *h, int subs, double *f1d) {
+  if (need_lattice(h)) return 1;
+  if (subs < 0 || subs >= h->p.nsubs) return lbm_fail("bad subs %d", subs);
+  if (!f1d) return lbm_fail("null population array");
+  const long elems = (long)h->Q * h->N;
+  if (elems > kScratchDoubles)
+    return lbm_fail("lbm_download_f_noalloc: Q x N = %ld doubles exceed the %ld of its static staging array "
+                    "(use lbm_download_f)",
+                    elems, kScratchDoubles);
+  double *stage = h->scratch;
+  int rc = dispatch(h, [&]<int Q, int NS, typename ST, typename CT>() -> int {
+    StepArgs<ST, CT> A = make_args<ST, CT>(h);
+    const dim3 g(blocks_for(h->N)), b(256);
+    if (subs == 0) hipLaunchKernelGGL((k_decode_dense<Q, NS, ST, CT, 0>), g, b, 0, h->stream, A, stage, (int)h->N);
+    else if constexpr (NS == 2)
+      hipLaunchKernelGGL((k_decode_dense<Q, NS, ST, CT, 1>), g, b, 0, h->stream, A, stage, (int)h->N);
+    HIP_OK(hipGetLastError());
+    return 0;
+  });
+  if (rc) return rc;
+  HIP_OK(hipMemcpyAsync(f1d, stage, (size_t)elems * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  return 0;
+}
 
 extern "C" int lbm_init_rest(lbm_handle *h, const double *c0) {
   if (need_lattice(h)) return 1;

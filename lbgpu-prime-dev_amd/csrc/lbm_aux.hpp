@@ -27,6 +27,24 @@ __global__ void k_decode(StepArgs<ST, CT> A, double *stage) {
                                                              shift_of<Q, ST, CT, SUBS>(A, a));
 }
 
+// one distribution in the boundary's dense [Q][N] layout: thread n takes
+// dense node n, finds its compact index from the row words (-1: solid) and
+// writes its Q slots (0 at a solid node), so every element of out is
+// written and nothing of an earlier call's contents survives
+template <int Q, int NS, typename ST, typename CT, int SUBS>
+__global__ void k_decode_dense(StepArgs<ST, CT> A, double *out, int n_nodes) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_nodes) return;
+  const int r = (int)fdiv((uint32_t)n, A.G.div_ni);
+  const int idx = rank_of(A.G, n - r * A.G.ni, r);
+#pragma unroll
+  for (int a = 0; a < Q; a++)
+    out[(long)a * n_nodes + n] =
+        idx < 0 ? 0.0
+                : (double)Codec<ST, CT>::dec(A.F[foff<Q, NS>(SUBS, a, idx)], A.inv_scale[SUBS],
+                                             shift_of<Q, ST, CT, SUBS>(A, a));
+}
+
 // rest state: f~ = w rho (3 c.u + 4.5 (c.u)^2 - 1.5 u^2) = 0 at u = 0;
 // solute g = w c0 (src/latman.c:1786-1893)
 template <int Q, int NS, typename ST, typename CT>
