@@ -79,8 +79,10 @@ struct lbm_handle {
   long tiles = 0;  // kTile-node tiles incl. ghost entries
   int W = 0;       // 64-node words per lattice row
   int nblk8 = 0;  // blocks of kBlock threads / 8
-  int *bmap = nullptr;  // row-band block map of the odd half-step (kBlockOdd threads)
+  int *bmap = nullptr;  // row-band block map of the odd half-step (its workgroup size)
   int bmap8 = 0;        // its length / 8
+  int bmap_bs = 0;      // the node block it was built for
+  int lds_cu = 0;       // LDS per CU of the device (bytes)
   int *amap = nullptr;  // row-band block map of the adjoint step (kBlock threads)
   size_t st_bytes = 8;
   void *F = nullptr;
@@ -352,7 +354,7 @@ void free_lattice(lbm_handle *h) {
   h->halo_pos = h->halo_neg = nullptr;
   h->ns = nullptr;
   h->bmap = h->amap = nullptr;
-  h->bmap8 = 0;
+  h->bmap8 = h->bmap_bs = 0;
   h->nf = 0;
 }
 
@@ -458,9 +460,18 @@ int upload_band_map(lbm_handle *h, int bs, long nb, long per, int **dst) {
   return 0;
 }
 
-// The 3-D lattice's block maps.  Odd half-step (blocks of odd_block(): 512
-// nodes, 256 with FP16 storage): only when an XCD's band of one plane holds
-// at least 3 blocks.  Measured on sphere packs
+// the odd half-step's workgroup size and LDS reservation for the handle's kind
+int odd_block(const lbm_handle *h) {
+  return dispatch(h, [&]<int Q, int NS, typename ST, typename CT>() -> int { return OddShape<NS, ST, CT>::block; });
+}
+template <int NS, typename ST, typename CT>
+size_t odd_pad(const lbm_handle *h) {
+  return odd_lds_pad(OddShape<NS, ST, CT>::per_cu, h->lds_cu);
+}
+
+// The 3-D lattice's block maps.  Odd half-step (blocks of odd_block()):
+// only when an XCD's band of one plane holds at least 1536 nodes (3 blocks
+// of 512, the size the threshold was measured with).  Measured on sphere packs
 // of porosity 0.40, odd + even step: 256^3 (6.7 blocks) flow FP64 0.777 ->
 // 0.727 ms, flow + solute FP64 1.591 -> 1.497 ms, FP32 0.785 -> 0.741 ms;
 // 192^3 (3.9 blocks) 1 to 3 % faster; 128^3 (1.7 blocks) 2 to 4 % slower,
@@ -469,12 +480,13 @@ int upload_band_map(lbm_handle *h, int bs, long nb, long per, int **dst) {
 // written): at 128^3 0.319 -> 0.305 s of adjoint kernels per sweep.
 int build_band_maps(lbm_handle *h) {
   // the odd step's node block for this storage (FP16: 256 nodes)
-  const int bso = odd_block((int)h->st_bytes);
+  const int bso = odd_block(h);
   const long nbo = blocks_for(h->nf, bso);
-  if ((double)h->nf / h->p.nk / 8.0 >= 3.0 * bso) {
+  if ((double)h->nf / h->p.nk / 8.0 >= 3.0 * kBlockOdd) {
     const long per = (nbo + 7) / 8;
     if (int rc = upload_band_map(h, bso, nbo, per, &h->bmap)) return rc;
     h->bmap8 = (int)per;
+    h->bmap_bs = bso;
   }
   return upload_band_map(h, kBlock, 8L * h->nblk8, h->nblk8, &h->amap);
 }
@@ -485,7 +497,7 @@ template <typename ST, typename CT>
 dim3 odd_grid(const lbm_handle *h, StepArgs<ST, CT> &A, int bs) {
   A.nblk8 = (int)((blocks_for(h->nf, bs) + 7) / 8);
   A.bmap = nullptr;
-  if (bs == odd_block((int)h->st_bytes) && h->bmap) {
+  if (bs == h->bmap_bs && h->bmap) {
     A.bmap = h->bmap;
     A.nblk8 = h->bmap8;
   }
@@ -500,28 +512,29 @@ int launch_half(lbm_handle *h, int odd, int t, int write_mv, bool fuse_bc = fals
     A.solute_on = NS == 2 && t >= h->p.sigma_t_on && t <= h->p.sigma_t_off;
     Prof pr(h, odd ? 0 : 1);
     const bool off32 = !h->off64 && (size_t)h->tiles * NS * Q * kTile * sizeof(ST) <= 0xffffffffull;
-    const int bs = odd ? half_step_block<true, ST>() : half_step_block<false, ST>();
+    const int bs = odd ? half_step_block<true, NS, ST, CT>() : half_step_block<false, NS, ST, CT>();
     A.nblk8 = (int)((blocks_for(h->nf, bs) + 7) / 8);
     const dim3 grid = odd ? odd_grid(h, A, bs) : dim3(8 * A.nblk8), block(bs);
+    const size_t lds = odd ? odd_pad<NS, ST, CT>(h) : 0;
     if (h->ns) {
       // Walsh partial bounce-back
       A.ns = h->ns;
       if (odd && off32)
-        hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, true, true, true>), grid, block, 0, h->stream, A);
+        hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, true, true, true>), grid, block, lds, h->stream, A);
       else if (odd)
-        hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, true, false, true>), grid, block, 0, h->stream, A);
+        hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, true, false, true>), grid, block, lds, h->stream, A);
       else if (off32)
-        hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, false, true, true>), grid, block, 0, h->stream, A);
+        hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, false, true, true>), grid, block, lds, h->stream, A);
       else
-        hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, false, false, true>), grid, block, 0, h->stream, A);
+        hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, false, false, true>), grid, block, lds, h->stream, A);
     } else if (odd && off32)
-      hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, true, true>), grid, block, 0, h->stream, A);
+      hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, true, true>), grid, block, lds, h->stream, A);
     else if (odd)
-      hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, true, false>), grid, block, 0, h->stream, A);
+      hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, true, false>), grid, block, lds, h->stream, A);
     else if (off32)
-      hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, false, true>), grid, block, 0, h->stream, A);
+      hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, false, true>), grid, block, lds, h->stream, A);
     else
-      hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, false, false>), grid, block, 0, h->stream, A);
+      hipLaunchKernelGGL((k_half_step<Q, NS, ST, CT, false, false>), grid, block, lds, h->stream, A);
     HIP_OK(hipGetLastError());
     return 0;
   });
@@ -756,6 +769,14 @@ extern "C" int lbm_create(const lbm_params *p, lbm_handle **out) {
     const std::string why = g_err;
     lbm_destroy(h);
     return lbm_fail("%s", why.c_str());
+  }
+  {
+    // LDS per CU, for the odd half-step's cap on resident workgroups (none
+    // when the device does not report it)
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess)
+      h->lds_cu = v;
   }
   if (p->c_bot_map && p->dims == 3) {
     size_t E = (size_t)h->E;
@@ -1731,16 +1752,17 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
       if (fz) set_zb(h, A);
       A.Fo = (double *)seg[q];
       A.solute_on = 1;
-      const int bs = (t & 1) ? half_step_block<true, double>() : half_step_block<false, double>();
+      const int bs = (t & 1) ? half_step_block<true, 2, double, double>() : half_step_block<false, 2, double, double>();
+      const size_t lds = (t & 1) ? odd_pad<2, double, double>(h) : 0;
       A.nblk8 = (int)((blocks_for(h->nf, bs) + 7) / 8);
       const dim3 grid = (t & 1) ? odd_grid(h, A, bs) : dim3(8 * A.nblk8), block(bs);
       Prof pr(h, (t & 1) ? 0 : 1);
       if (t & 1) {
-        if (off32) hipLaunchKernelGGL((k_half_step<19, 2, double, double, true, true>), grid, block, 0, h->stream, A);
-        else hipLaunchKernelGGL((k_half_step<19, 2, double, double, true, false>), grid, block, 0, h->stream, A);
+        if (off32) hipLaunchKernelGGL((k_half_step<19, 2, double, double, true, true>), grid, block, lds, h->stream, A);
+        else hipLaunchKernelGGL((k_half_step<19, 2, double, double, true, false>), grid, block, lds, h->stream, A);
       } else {
-        if (off32) hipLaunchKernelGGL((k_half_step<19, 2, double, double, false, true>), grid, block, 0, h->stream, A);
-        else hipLaunchKernelGGL((k_half_step<19, 2, double, double, false, false>), grid, block, 0, h->stream, A);
+        if (off32) hipLaunchKernelGGL((k_half_step<19, 2, double, double, false, true>), grid, block, lds, h->stream, A);
+        else hipLaunchKernelGGL((k_half_step<19, 2, double, double, false, false>), grid, block, lds, h->stream, A);
       }
       if ((e = hipGetLastError()) != hipSuccess) fail("replay", e);
     }

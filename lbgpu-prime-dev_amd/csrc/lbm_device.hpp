@@ -37,11 +37,33 @@ constexpr int kBlock = 256;
 // compared with the contiguous order; 512 threads in the row-band order
 // gained 4 % at 256^3 but lost 4 % at 192^3.  The even step is best at 256.
 constexpr int kBlockOdd = 512;
-template <bool ODD, typename ST>
-constexpr int half_step_block() { return ODD && sizeof(ST) >= 4 ? kBlockOdd : kBlock; }
-// the odd half-step's node block for a storage type (the row-band block map
-// of a lattice is built for it)
-inline int odd_block(int storage_bytes) { return storage_bytes >= 4 ? kBlockOdd : kBlock; }
+// Launch shape of the odd half-step of one kernel instantiation: threads per
+// workgroup, and the number of its workgroups one CU may hold (0: as many as
+// the registers allow; otherwise each workgroup reserves LDS it does not use,
+// lds_per_cu / (per_cu + 1/2) rounded down, so that no more fit).
+// FP64 flow + solute (16-byte pair accesses, 136 registers per lane: at
+// most three waves per SIMD) runs one-wave workgroups, at most 6 per CU
+// (1.5 waves per SIMD).  At 256^3 in the row-band order its odd step took
+// 0.797 ms, against 0.830 ms with 512-thread workgroups (two waves per SIMD,
+// one workgroup per CU) and 0.841 ms with 256 (three per SIMD); capped at 8
+// one-wave workgroups per CU 0.806, at 4 0.833, at 10 0.814 ms.  With fewer
+// waves each holding 19 KB of pair requests the XCD's L2 serves the shared
+// lines before they leave it.  The other kinds keep 512 (256 with FP16)
+// threads and no cap: capping them to 1.5-2 waves per SIMD made them 7-45 %
+// slower.
+template <int NS, typename ST, typename CT>
+struct OddShape {
+  static constexpr bool one_wave = NS == 2 && sizeof(ST) == 8 && sizeof(CT) == 8;
+  static constexpr int block = one_wave ? 64 : (sizeof(ST) >= 4 ? kBlockOdd : kBlock);
+  static constexpr int per_cu = one_wave ? 6 : 0;
+};
+template <bool ODD, int NS, typename ST, typename CT>
+constexpr int half_step_block() { return ODD ? OddShape<NS, ST, CT>::block : kBlock; }
+// LDS a workgroup of the odd half-step reserves to cap the workgroups per CU
+inline size_t odd_lds_pad(int per_cu, int lds_per_cu) {
+  if (per_cu <= 0 || lds_per_cu <= 0) return 0;
+  return (size_t)(lds_per_cu / (per_cu + 0.5)) / 2048 * 2048;
+}
 
 // D3Q19 velocity set in the reference's order (src/lbgpu_prime.h:78-84):
 // opposite directions adjacent, D2Q9 first.
@@ -564,12 +586,12 @@ __device__ __forceinline__ void walsh_mix(CT (&f)[Q], const CT (&fp)[Q], CT ns, 
 }
 
 template <int Q, int NS, typename ST, typename CT, bool ODD, bool OFF32, bool WALSH = false>
-__global__ __launch_bounds__((half_step_block<ODD, ST>())) void k_half_step(StepArgs<ST, CT> A) {
+__global__ __launch_bounds__((half_step_block<ODD, NS, ST, CT>())) void k_half_step(StepArgs<ST, CT> A) {
   // node block: from the row-band map when the launch carries one, else the
   // XCD-contiguous remap
   const int lb = A.bmap ? A.bmap[blockIdx.x] : xcd_block(A.nblk8);
   if (lb < 0) return;
-  const int idx = lb * half_step_block<ODD, ST>() + (int)threadIdx.x;
+  const int idx = lb * half_step_block<ODD, NS, ST, CT>() + (int)threadIdx.x;
   if (idx >= A.nf) return;
   int nb[Q];
   nb[0] = idx;
