@@ -460,13 +460,16 @@ int upload_band_map(lbm_handle *h, int bs, long nb, long per, int **dst) {
   return 0;
 }
 
-// the odd half-step's workgroup size and LDS reservation for the handle's kind
+// the odd half-step's workgroup size for the handle's kind, and the LDS a
+// half-step workgroup reserves
 int odd_block(const lbm_handle *h) {
-  return dispatch(h, [&]<int Q, int NS, typename ST, typename CT>() -> int { return OddShape<NS, ST, CT>::block; });
+  return dispatch(h, [&]<int Q, int NS, typename ST, typename CT>() -> int {
+    return StepShape<true, NS, ST, CT>::block;
+  });
 }
-template <int NS, typename ST, typename CT>
-size_t odd_pad(const lbm_handle *h) {
-  return odd_lds_pad(OddShape<NS, ST, CT>::per_cu, h->lds_cu);
+template <bool ODD, int NS, typename ST, typename CT>
+size_t step_pad(const lbm_handle *h) {
+  return step_lds_pad(StepShape<ODD, NS, ST, CT>::per_cu, h->lds_cu);
 }
 
 // The 3-D lattice's block maps.  Odd half-step (blocks of odd_block()):
@@ -515,7 +518,7 @@ int launch_half(lbm_handle *h, int odd, int t, int write_mv, bool fuse_bc = fals
     const int bs = odd ? half_step_block<true, NS, ST, CT>() : half_step_block<false, NS, ST, CT>();
     A.nblk8 = (int)((blocks_for(h->nf, bs) + 7) / 8);
     const dim3 grid = odd ? odd_grid(h, A, bs) : dim3(8 * A.nblk8), block(bs);
-    const size_t lds = odd ? odd_pad<NS, ST, CT>(h) : 0;
+    const size_t lds = odd ? step_pad<true, NS, ST, CT>(h) : step_pad<false, NS, ST, CT>(h);
     if (h->ns) {
       // Walsh partial bounce-back
       A.ns = h->ns;
@@ -1753,7 +1756,7 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
       A.Fo = (double *)seg[q];
       A.solute_on = 1;
       const int bs = (t & 1) ? half_step_block<true, 2, double, double>() : half_step_block<false, 2, double, double>();
-      const size_t lds = (t & 1) ? odd_pad<2, double, double>(h) : 0;
+      const size_t lds = (t & 1) ? step_pad<true, 2, double, double>(h) : step_pad<false, 2, double, double>(h);
       A.nblk8 = (int)((blocks_for(h->nf, bs) + 7) / 8);
       const dim3 grid = (t & 1) ? odd_grid(h, A, bs) : dim3(8 * A.nblk8), block(bs);
       Prof pr(h, (t & 1) ? 0 : 1);

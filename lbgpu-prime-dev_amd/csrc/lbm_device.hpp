@@ -37,30 +37,39 @@ constexpr int kBlock = 256;
 // compared with the contiguous order; 512 threads in the row-band order
 // gained 4 % at 256^3 but lost 4 % at 192^3.  The even step is best at 256.
 constexpr int kBlockOdd = 512;
-// Launch shape of the odd half-step of one kernel instantiation: threads per
-// workgroup, and the number of its workgroups one CU may hold (0: as many as
-// the registers allow; otherwise each workgroup reserves LDS it does not use,
+// Launch shape of a half-step kernel instantiation: threads per workgroup,
+// and the number of its workgroups one CU may hold (0: as many as the
+// registers allow; otherwise each workgroup reserves LDS it does not use,
 // lds_per_cu / (per_cu + 1/2) rounded down, so that no more fit).
-// FP64 flow + solute (16-byte pair accesses, 136 registers per lane: at
-// most three waves per SIMD) runs one-wave workgroups, at most 6 per CU
-// (1.5 waves per SIMD).  At 256^3 in the row-band order its odd step took
-// 0.797 ms, against 0.830 ms with 512-thread workgroups (two waves per SIMD,
-// one workgroup per CU) and 0.841 ms with 256 (three per SIMD); capped at 8
-// one-wave workgroups per CU 0.806, at 4 0.833, at 10 0.814 ms.  With fewer
-// waves each holding 19 KB of pair requests the XCD's L2 serves the shared
-// lines before they leave it.  The other kinds keep 512 (256 with FP16)
-// threads and no cap: capping them to 1.5-2 waves per SIMD made them 7-45 %
-// slower.
-template <int NS, typename ST, typename CT>
-struct OddShape {
-  static constexpr bool one_wave = NS == 2 && sizeof(ST) == 8 && sizeof(CT) == 8;
-  static constexpr int block = one_wave ? 64 : (sizeof(ST) >= 4 ? kBlockOdd : kBlock);
-  static constexpr int per_cu = one_wave ? 6 : 0;
+// With FP64 storage and collision the half-steps run at low occupancy in
+// small workgroups: fewer requests in flight per XCD let its L2 serve the
+// lines that neighbouring waves share (and the read-then-write of each line)
+// before they leave it, and small workgroups retire wave by wave instead of
+// waiting for their slowest wave.  Measured at 256^3 (porosity 0.40, odd
+// step in the row-band order), odd / even half-step:
+//   flow + solute: odd in one-wave workgroups, at most 6 per CU (1.5 waves
+//   per SIMD): 0.793 ms against 0.830 with 512 threads (two waves per SIMD,
+//   registers allow no more), 0.841 with 256 (three per SIMD), 0.806 / 0.833
+//   / 0.814 with at most 8 / 4 / 10 one-wave workgroups; even in two-wave
+//   workgroups, at most 2 per CU (one wave per SIMD): 0.642 ms against 0.666
+//   with 256 threads uncapped, 0.648 with one-wave workgroups at most 3-6
+//   per CU, 0.659 with 256 threads at most 2-3 per CU;
+//   flow: odd stays at 512 threads uncapped (0.395 ms; capped or smaller,
+//   0.42-0.54); even in two-wave workgroups at most 3 per CU: 0.328 against
+//   0.338 ms.
+// FP32 and FP16 storage keep 512 / 256 threads uncapped: the same caps made
+// them 3-70 % slower.
+template <bool ODD, int NS, typename ST, typename CT>
+struct StepShape {
+  static constexpr bool f64 = sizeof(ST) == 8 && sizeof(CT) == 8;
+  static constexpr int block =
+      ODD ? (NS == 2 && f64 ? 64 : (sizeof(ST) >= 4 ? kBlockOdd : kBlock)) : (f64 ? 128 : kBlock);
+  static constexpr int per_cu = !f64 ? 0 : (ODD ? (NS == 2 ? 6 : 0) : (NS == 2 ? 2 : 3));
 };
 template <bool ODD, int NS, typename ST, typename CT>
-constexpr int half_step_block() { return ODD ? OddShape<NS, ST, CT>::block : kBlock; }
-// LDS a workgroup of the odd half-step reserves to cap the workgroups per CU
-inline size_t odd_lds_pad(int per_cu, int lds_per_cu) {
+constexpr int half_step_block() { return StepShape<ODD, NS, ST, CT>::block; }
+// LDS a half-step workgroup reserves to cap the workgroups per CU
+inline size_t step_lds_pad(int per_cu, int lds_per_cu) {
   if (per_cu <= 0 || lds_per_cu <= 0) return 0;
   return (size_t)(lds_per_cu / (per_cu + 0.5)) / 2048 * 2048;
 }
