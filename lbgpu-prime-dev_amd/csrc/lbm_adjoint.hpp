@@ -53,6 +53,15 @@ __device__ __forceinline__ uint32_t push_off(const int (&nb)[19], int s, int a, 
   return foff<19, 2>(s, a, idx);
 }
 
+// Threads per workgroup of the adjoint step: one wave.  With one or two
+// waves per SIMD (the register counts below) a workgroup of four waves
+// holds its CU's slots until its slowest wave ends; one-wave workgroups are
+// replaced wave by wave (kernels of a 128^3 sweep: 289 ms against 305 ms
+// with 256 threads, 291 ms with 128; capping the workgroups per CU gained
+// nothing).  The gradient partials are per node block, so the block size
+// fixes the order of their sums.
+constexpr int kAdjBlock = 64;
+
 // Waves per SIMD the register allocation aims at.  The even step at two
 // waves (about 236 registers): squeezed to three it fits in 150 registers
 // only by reusing one register quadruple for all 19 stores, and each store
@@ -249,7 +258,7 @@ __device__ __forceinline__ void adj_neighbours(const StepArgs<double, double> &A
 __device__ __forceinline__ void adj_block_sum(const AdjArgs &J, long slot, double (&acc)[5]) {
   __shared__ double red[5];
   const int kinds[5] = {0, 0, 0, 0, 0};
-  block_reduce<5>(acc, kinds, red);
+  block_reduce<5, kAdjBlock / 64>(acc, kinds, red);
   if (threadIdx.x < 5) J.gacc[slot * 5 + threadIdx.x] += red[threadIdx.x];
 }
 
@@ -344,7 +353,7 @@ __device__ __forceinline__ double adj_zface_solute(bool top, double (&l)[19]) {
 template <bool ODD, bool OFF32>
 __device__ __forceinline__ void adj_face_block(const AdjArgs &J, long slot, int fb) {
   const StepArgs<double, double> &A = J.A;
-  const int t = fb * kBlock + (int)threadIdx.x;
+  const int t = fb * kAdjBlock + (int)threadIdx.x;
   const int nlo = A.zb.lo_end, nface = nlo + (A.nf - A.zb.hi_begin);
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (t < nface) {
@@ -397,14 +406,14 @@ __device__ __forceinline__ void adj_face_block(const AdjArgs &J, long slot, int 
 // Gradient partials: slot b of face block b, slot nfb + node block of the
 // others (so the block order does not change the sums).
 template <bool ODD, bool OFF32>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ODD ? LBM_ADJ_WAVES_ODD : LBM_ADJ_WAVES_EVEN)))
+__global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(ODD ? LBM_ADJ_WAVES_ODD : LBM_ADJ_WAVES_EVEN)))
 void k_adjoint_step(AdjArgs J, int nfb) {
   if ((int)blockIdx.x < nfb) {
     adj_face_block<ODD, OFF32>(J, blockIdx.x, blockIdx.x);
     return;
   }
   const int lb = J.amap ? J.amap[blockIdx.x - nfb] : xcd_block(J.A.nblk8, (int)blockIdx.x - nfb);
-  const int idx = lb * kBlock + (int)threadIdx.x;
+  const int idx = lb * kAdjBlock + (int)threadIdx.x;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (idx < J.A.nf && !zface_node<19>(J.A, idx)) {
     int nb[19], k;

@@ -79,11 +79,12 @@ struct lbm_handle {
   long tiles = 0;  // kTile-node tiles incl. ghost entries
   int W = 0;       // 64-node words per lattice row
   int nblk8 = 0;  // blocks of kBlock threads / 8
+  int anblk8 = 0; // blocks of kAdjBlock threads / 8
   int *bmap = nullptr;  // row-band block map of the odd half-step (its workgroup size)
   int bmap8 = 0;        // its length / 8
   int bmap_bs = 0;      // the node block it was built for
   int lds_cu = 0;       // LDS per CU of the device (bytes)
-  int *amap = nullptr;  // row-band block map of the adjoint step (kBlock threads)
+  int *amap = nullptr;  // row-band block map of the adjoint step (kAdjBlock threads)
   size_t st_bytes = 8;
   void *F = nullptr;
   void *Fout = nullptr;  // where the half-steps write when not F (out-of-place steps)
@@ -479,10 +480,10 @@ size_t step_pad(const lbm_handle *h) {
 // 0.727 ms, flow + solute FP64 1.591 -> 1.497 ms, FP32 0.785 -> 0.741 ms;
 // 192^3 (3.9 blocks) 1 to 3 % faster; 128^3 (1.7 blocks) 2 to 4 % slower,
 // the even step that follows losing more than the odd step gains.  Adjoint
-// step (kBlock, all 8 * nblk8 blocks so that every gradient slot is
+// step (kAdjBlock, all 8 * anblk8 blocks so that every gradient slot is
 // written): at 128^3 0.319 -> 0.305 s of adjoint kernels per sweep.
 int build_band_maps(lbm_handle *h) {
-  // the odd step's node block for this storage (FP16: 256 nodes)
+  // the odd step's node block for this kind
   const int bso = odd_block(h);
   const long nbo = blocks_for(h->nf, bso);
   if ((double)h->nf / h->p.nk / 8.0 >= 3.0 * kBlockOdd) {
@@ -491,7 +492,7 @@ int build_band_maps(lbm_handle *h) {
     h->bmap8 = (int)per;
     h->bmap_bs = bso;
   }
-  return upload_band_map(h, kBlock, 8L * h->nblk8, h->nblk8, &h->amap);
+  return upload_band_map(h, kAdjBlock, 8L * h->anblk8, h->anblk8, &h->amap);
 }
 
 // the odd half-step's grid: the row-band map when there is one for this
@@ -834,6 +835,7 @@ static int build_lattice(lbm_handle *h, const uint8_t *mask) {
   h->tiles = (slots + kTile - 1) / kTile;
   long nblk = blocks_for(nf, kBlock);
   h->nblk8 = (int)((nblk + 7) / 8);
+  h->anblk8 = (int)((blocks_for(nf, kAdjBlock) + 7) / 8);
   if (h->D == 3)
     if (int rc = build_band_maps(h)) return rc;
   const int Q = h->Q, ns = h->p.nsubs;
@@ -1684,7 +1686,7 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
   const int nrep = (h->obj.kept_from + K - 1) / K;
   const int saved_time = h->time;
   void *const saved_F = h->F;  // the replay points h->F at segment buffers
-  const int nblk = 8 * h->nblk8;
+  const int nblk = 8 * h->anblk8;
   int rc = 0;
   auto fail = [&](const char *what, hipError_t e) {
     rc = lbm_fail("adjoint: %s: %s", what, hipGetErrorString(e));
@@ -1708,7 +1710,7 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
   // gradient partials: one slot per block of the step kernel, then one per
   // block of the face-node kernel
   // face blocks of the fused adjoint step: a multiple of 8 (XCD mapping)
-  const int nfb = (int)((blocks_for((long)h->n_lo + h->n_hi, kBlock) + 7) / 8 * 8);
+  const int nfb = (int)((blocks_for((long)h->n_lo + h->n_hi, kAdjBlock) + 7) / 8 * 8);
   const int nslot = nblk + nfb;
   if (!rc && !h->obj.gacc && (e = hipMalloc(&h->obj.gacc, (size_t)nslot * 5 * sizeof(double))) != hipSuccess)
     fail("gradients", e);
@@ -1778,6 +1780,7 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
       const bool fz = zface_fused(h, t);
       J.A = make_args<double, double>(h);
       J.amap = h->amap;
+      J.A.nblk8 = h->anblk8;
       if (fz) set_zb(h, J.A);
       J.L = L;
       J.seed = t > t0 + T - W_ ? seed : 0.0;
@@ -1809,7 +1812,7 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
       // nodes (their conditions formed from the state, transposed before the
       // store); otherwise every node, and the transposes follow
       const int fb = fz ? nfb : 0;
-      dim3 grid(nblk + fb), block(kBlock);
+      dim3 grid(nblk + fb), block(kAdjBlock);
       Prof pr(h, 5);
       if (odd) {
         if (off32) hipLaunchKernelGGL((k_adjoint_step<true, true>), grid, block, 0, h->stream, J, fb);

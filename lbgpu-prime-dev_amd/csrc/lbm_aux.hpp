@@ -133,11 +133,11 @@ __device__ __forceinline__ double wave_min(double v) {
   return v;
 }
 
-// kind: 0 sum, 1 max, 2 min.  Reduces K values over the block (256
-// threads = 4 waves) in a fixed order.
-template <int K>
+// kind: 0 sum, 1 max, 2 min.  Reduces K values over the block (NW waves:
+// 256 threads unless stated) in a fixed order.
+template <int K, int NW = 4>
 __device__ void block_reduce(double (&v)[K], const int (&kind)[K], double *out) {
-  __shared__ double lds[4][K];
+  __shared__ double lds[NW][K];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < K; q++) {
@@ -148,7 +148,7 @@ __device__ void block_reduce(double (&v)[K], const int (&kind)[K], double *out) 
   if (threadIdx.x < K) {
     int q = threadIdx.x;
     double x = lds[0][q];
-    for (int w = 1; w < 4; w++)
+    for (int w = 1; w < NW; w++)
       x = kind[q] == 0 ? x + lds[w][q] : (kind[q] == 1 ? fmax(x, lds[w][q]) : fmin(x, lds[w][q]));
     out[q] = x;
   }

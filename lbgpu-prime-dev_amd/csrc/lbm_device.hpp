@@ -101,7 +101,9 @@ __host__ __device__ constexpr double W(int a) {
 template <typename ST, typename CT>
 struct Codec {
   static __device__ __forceinline__ CT dec(ST v, CT inv_scale, CT shift) {
-    return (CT)v * inv_scale + shift;
+    // FP32 collision: one fused multiply-add (no FP64 parity to keep)
+    if constexpr (sizeof(CT) == 4) return __builtin_fmaf((CT)v, inv_scale, shift);
+    else return (CT)v * inv_scale + shift;
   }
   static __device__ __forceinline__ ST enc(CT v, CT scale, CT shift) { return (ST)((v - shift) * scale); }
 };
@@ -220,11 +222,68 @@ __device__ __forceinline__ void gather(const StepArgs<ST, CT> &A, int idx, const
   }
 }
 
+// ---- FP16 stores with stochastic rounding ----------------------------------
+// Rounded to nearest, a population that changes little from one step to the
+// next rounds the same way at every step and the errors add up (192^3 flow +
+// solute, 4000 steps: the density drifted by 1e-4 in nearly closed pockets,
+// the concentration behind the front came out up to 4 % low).  The
+// half-steps therefore round each stored FP16 value up or down with
+// probabilities given by its distance to the two neighbouring FP16 values,
+// which makes every rounding unbiased (the errors then add like a random
+// walk).  gfx950's v_cvt_sr_f16_f32 does the rounding in one instruction,
+// from the top bits of a 32-bit random operand (checked on the device with
+// 8192 operands per value: with the operand's top 13 bits uniform the mean
+// equals the input to the last bit, subnormal results included, and values
+// near 65504 overflow to infinity with the right probability, as round to
+// nearest would past 65520; with only its low bits set it truncates).  The
+// random operand of a value is the low 13 mantissa bits of another value of
+// the same node (its FP32 rounding residue, which the value's own rounding
+// does not see) moved to the top, plus a per-node seed hashed from the node
+// index and the node's post-collision rest populations: one shift-add per
+// value (32-bit multiplies per slot made the FP16 even half-step 10 %
+// slower).  The stored values thus depend on the node's state only, not on
+// the block order or the launch path (captured graph or step by step), and
+// a node whose state does not change keeps storing the same values.
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+  // integer finaliser "lowbias32" (C. Wellons, hash-prospector)
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+__device__ __forceinline__ uint32_t sr_seed(int idx, float r0, float r1) {
+  const uint32_t b1 = __float_as_uint(r1);
+  return mix32((uint32_t)idx * 0x9e3779b9u ^ __float_as_uint(r0) ^ ((b1 << 13) | (b1 >> 19)));
+}
+// random operand for a value: the low 13 mantissa bits of `other` moved to
+// the top, plus the seed
+__device__ __forceinline__ uint32_t sr_rand(float other, uint32_t seed) {
+  return (__float_as_uint(other) << 19) + seed;
+}
+typedef _Float16 half2_sr __attribute__((ext_vector_type(2)));
+// x and y to FP16 with stochastic rounding, each from the other's residue
+__device__ __forceinline__ half2_sr sr_pair(float x, float y, uint32_t seed) {
+  half2_sr v = {(_Float16)0, (_Float16)0};
+  v = __builtin_amdgcn_cvt_sr_f16_f32(v, x, sr_rand(y, seed), false);
+  v = __builtin_amdgcn_cvt_sr_f16_f32(v, y, sr_rand(x, (seed << 16) | (seed >> 16)), true);
+  return v;
+}
+__device__ __forceinline__ _Float16 sr_one(float x, uint32_t r) {
+  half2_sr v = {(_Float16)0, (_Float16)0};
+  v = __builtin_amdgcn_cvt_sr_f16_f32(v, x, r, false);
+  return v.x;
+}
+
 template <int Q, int NS, typename ST, typename CT, int SUBS, bool ODD, bool OFF32 = false>
 __device__ __forceinline__ void scatter(const StepArgs<ST, CT> &A, int idx, const int (&nb)[Q],
                                         const CT (&f)[Q]) {
+  [[maybe_unused]] uint32_t seed = 0;
+  if constexpr (sizeof(ST) == 2) seed = sr_seed(idx, (float)f[0], (float)f[1]);
 #pragma unroll
   for (int a = 0; a < Q; a++) {
+    [[maybe_unused]] constexpr int kPartner[19] = {1, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
     uint32_t off;
     if (a == 0)
       off = foff<Q, NS>(SUBS, 0, idx);
@@ -232,8 +291,16 @@ __device__ __forceinline__ void scatter(const StepArgs<ST, CT> &A, int idx, cons
       off = nb[OPP(a)] >= 0 ? foff<Q, NS>(SUBS, OPP(a), nb[OPP(a)]) : foff<Q, NS>(SUBS, a, idx);
     else
       off = foff<Q, NS>(SUBS, a, idx);
-    st_stream<!ODD>(
-        elem<OFF32>(A.Fo, off), Codec<ST, CT>::enc(f[a], A.scale[SUBS], shift_of<Q, ST, CT, SUBS>(A, a)));
+    if constexpr (sizeof(ST) == 2) {
+      // the partner's value: another slot of the node (its opposite)
+      const int b = kPartner[a];
+      const float y = (float)((f[a] - shift_of<Q, ST, CT, SUBS>(A, a)) * A.scale[SUBS]);
+      const float yb = (float)((f[b] - shift_of<Q, ST, CT, SUBS>(A, b)) * A.scale[SUBS]);
+      st_stream<!ODD>(elem<OFF32>(A.Fo, off), (ST)sr_one(y, sr_rand(yb, seed)));
+    } else {
+      st_stream<!ODD>(
+          elem<OFF32>(A.Fo, off), Codec<ST, CT>::enc(f[a], A.scale[SUBS], shift_of<Q, ST, CT, SUBS>(A, a)));
+    }
   }
 }
 
@@ -263,6 +330,8 @@ __device__ __forceinline__ void gather2(const StepArgs<ST, CT> &A, int idx, cons
 template <int Q, typename ST, typename CT, bool ODD, bool OFF32>
 __device__ __forceinline__ void scatter2(const StepArgs<ST, CT> &A, int idx, const int (&nb)[Q],
                                          const CT (&f)[Q], const CT (&g)[Q]) {
+  [[maybe_unused]] uint32_t seed = 0;
+  if constexpr (sizeof(ST) == 2) seed = sr_seed(idx, (float)f[0], (float)g[0]);
 #pragma unroll
   for (int a = 0; a < Q; a++) {
     uint32_t off;
@@ -273,8 +342,15 @@ __device__ __forceinline__ void scatter2(const StepArgs<ST, CT> &A, int idx, con
     else
       off = foff<Q, 2>(0, a, idx);
     pair_t<ST> v;
-    v.x = Codec<ST, CT>::enc(f[a], A.scale[0], (CT)0);
-    v.y = Codec<ST, CT>::enc(g[a], A.scale[1], shift_of<Q, ST, CT, 1>(A, a));
+    if constexpr (sizeof(ST) == 2) {
+      const half2_sr h = sr_pair((float)(f[a] * A.scale[0]),
+                                 (float)((g[a] - shift_of<Q, ST, CT, 1>(A, a)) * A.scale[1]), seed);
+      v.x = h.x;
+      v.y = h.y;
+    } else {
+      v.x = Codec<ST, CT>::enc(f[a], A.scale[0], (CT)0);
+      v.y = Codec<ST, CT>::enc(g[a], A.scale[1], shift_of<Q, ST, CT, 1>(A, a));
+    }
     st_stream<!ODD>((pair_t<ST> *)elem<OFF32>(A.Fo, off), v);
   }
 }
@@ -402,18 +478,37 @@ __device__ __forceinline__ void collide(const StepArgs<ST, CT> &A, int idx, CT (
       if (D == 3) vd[a] += ((CT)CZ(a)) * u[2];
     }
   }
+  if constexpr (sizeof(CT) == 4) {
+    // FP32 collision, the same update factored per node and fused
+    // (FP32 and FP16 storage have no FP64 parity to keep): flow
+    //   f (1 - it) + w [it (rho - 1.5 rho_A u.u) + it rho_A v (3 + 4.5 v)],
+    // solute  f (1 - it) + w it rho (1 + 3 v),  v = c_a . u
+    const CT k1 = (CT)1 - it;
+    const CT B = SUBS == 1 ? it * rho : it * (rho - (CT)1.5 * A.rhoA * usq);
+    const CT C = it * A.rhoA;
 #pragma unroll
-  for (int a = 0; a < Q; a++) {
-    CT fa = f[a] * ((CT)1 - it);
-    const CT vdotu = vd[a];
-    const CT w = (CT)W<Q>(a);
-    if (SUBS == 1) {
-      fa += w * rho * ((CT)1 + (CT)3 * vdotu) * it;
-    } else {
-      fa += w * rho * it;
-      fa += w * A.rhoA * ((CT)3 * vdotu + (CT)4.5 * vdotu * vdotu - (CT)1.5 * usq) * it;
+    for (int a = 0; a < Q; a++) {
+      const CT v = vd[a];
+      const CT w = (CT)W<Q>(a);
+      CT t;
+      if (SUBS == 1) t = B * __builtin_fmaf((CT)3, v, (CT)1);
+      else t = __builtin_fmaf(C, v * __builtin_fmaf((CT)4.5, v, (CT)3), B);
+      f[a] = __builtin_fmaf(f[a], k1, w * t);
     }
-    f[a] = fa;
+  } else {
+#pragma unroll
+    for (int a = 0; a < Q; a++) {
+      CT fa = f[a] * ((CT)1 - it);
+      const CT vdotu = vd[a];
+      const CT w = (CT)W<Q>(a);
+      if (SUBS == 1) {
+        fa += w * rho * ((CT)1 + (CT)3 * vdotu) * it;
+      } else {
+        fa += w * rho * it;
+        fa += w * A.rhoA * ((CT)3 * vdotu + (CT)4.5 * vdotu * vdotu - (CT)1.5 * usq) * it;
+      }
+      f[a] = fa;
+    }
   }
   if constexpr (sizeof(CT) == 4 && sizeof(ST) >= 4) {
     // FP32 collision (FP32 or FP64 storage; FP16 storage, whose rounding
