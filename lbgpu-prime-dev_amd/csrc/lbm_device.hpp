@@ -57,13 +57,19 @@ constexpr int kBlockOdd = 512;
 //   flow: odd stays at 512 threads uncapped (0.395 ms; capped or smaller,
 //   0.42-0.54); even in two-wave workgroups at most 3 per CU: 0.328 against
 //   0.338 ms.
-// FP32 and FP16 storage keep 512 / 256 threads uncapped: the same caps made
-// them 3-70 % slower.
+// FP32 storage with the FP64 collision (flow + solute; 141 / 178 registers:
+// at most three / two waves per SIMD) takes the small workgroups without a
+// cap: odd in one-wave workgroups 0.420 ms against 0.441 with 512 threads,
+// even in two-wave workgroups 0.307 against 0.331 with 256 (capped at 1.5
+// or 1 wave per SIMD both were slower: 0.46-0.70 / 0.32-0.36 ms).
+// FP32 storage with the FP32 collision and FP16 storage keep 512 / 256
+// threads uncapped: smaller workgroups or caps made them 1-70 % slower.
 template <bool ODD, int NS, typename ST, typename CT>
 struct StepShape {
   static constexpr bool f64 = sizeof(ST) == 8 && sizeof(CT) == 8;
-  static constexpr int block =
-      ODD ? (NS == 2 && f64 ? 64 : (sizeof(ST) >= 4 ? kBlockOdd : kBlock)) : (f64 ? 128 : kBlock);
+  static constexpr bool f32_f64 = NS == 2 && sizeof(ST) == 4 && sizeof(CT) == 8;
+  static constexpr int block = ODD ? (NS == 2 && (f64 || f32_f64) ? 64 : (sizeof(ST) >= 4 ? kBlockOdd : kBlock))
+                                   : (f64 || f32_f64 ? 128 : kBlock);
   static constexpr int per_cu = !f64 ? 0 : (ODD ? (NS == 2 ? 6 : 0) : (NS == 2 ? 2 : 3));
 };
 template <bool ODD, int NS, typename ST, typename CT>
