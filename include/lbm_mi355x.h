@@ -30,6 +30,12 @@ extern "C" {
 
 typedef struct lbm_handle lbm_handle;
 
+/* Population storage.  FP32 stores the shifted populations f - w rho_A;
+ * FP16 stores them scaled (store_scale), and the half-steps round each
+ * stored FP16 value stochastically (unbiased: up or down with probabilities
+ * set by its distance to the two neighbouring FP16 values, from a hash of
+ * the node's own state, so results are reproducible and independent of the
+ * launch order).  Collision runs in the compute type; reductions are FP64. */
 enum lbm_storage { LBM_STORE_F64 = 0, LBM_STORE_F32 = 1, LBM_STORE_F16 = 2 };
 enum lbm_compute { LBM_COMPUTE_F64 = 0, LBM_COMPUTE_F32 = 1 };
 /* what a step writes to the macroscopic buffer */
