@@ -304,8 +304,9 @@ int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap);
 /* Device memory the next objective runs may spend on states kept for the
  * adjoint (no reference counterpart; default 0): the run keeps the states
  * of its last whole segments of `every` steps, as many as max_bytes holds
- * (one state is the size of the population array) while every + 2 states
- * and 1 GiB of device memory stay free for the adjoint run, and the adjoint
+ * (one state is the size of the population array) while the checkpoints,
+ * every + 2 states and 1 GiB of device memory stay free for the adjoint run,
+ * and the adjoint
  * sweep uses them instead of replaying those segments from checkpoints.  The
  * gradient is the same bit for bit.  The kept states stay allocated like
  * the checkpoints. */

@@ -1533,12 +1533,14 @@ extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int ever
   long P = 0;
   if (h->obj.keep_bytes >= sb) {
     long cap = (long)(h->obj.keep_bytes / sb);
-    // no more than leaves room for the adjoint's own buffers (every + 2
-    // states) and 1 GiB besides, counting the pool already held
+    // no more than leaves room for the checkpoints (at most one per segment),
+    // the adjoint's own buffers (every + 2 states) and 1 GiB besides, counting
+    // the kept states and checkpoints already held
     size_t free_b = 0, total_b = 0;
     HIP_OK(hipMemGetInfo(&free_b, &total_b));
-    const double room = (double)free_b + (double)h->obj.kept.size() * (double)sb -
-                        (double)(every + 2) * (double)sb - (double)(1ull << 30);
+    const double held = (double)(h->obj.kept.size() + h->obj.ckpt.size()) * (double)sb;
+    const double ckpt_max = (double)((nsteps + every - 1) / every) * (double)sb;
+    const double room = (double)free_b + held - ckpt_max - (double)(every + 2) * (double)sb - (double)(1ull << 30);
     cap = std::min(cap, room > 0 ? (long)(room / (double)sb) : 0L);
     P = cap >= nsteps ? nsteps : nsteps - (long)(nsteps - cap + every - 1) / every * every;
     if (P < 0) P = 0;
