@@ -27,15 +27,14 @@
 namespace lbm {
 
 constexpr int kBlock = 256;
-// Workgroup size of the odd (pull/push) half-step.  Its 19 gathers per node
-// keep more requests in flight per CU with 8-wave workgroups: at 256^3
-// (D3Q19, FP64, porosity 0.40) the odd step takes 0.437 ms at 512 threads
-// against 0.457 ms at 256 and 0.491 ms at 1024 (contiguous block order).
-// With FP16 storage (flow + solute) it stays at 256 threads: in the
-// row-band order with 256-node blocks its odd step took 0.242 against
-// 0.267 ms at 256^3 and 0.119 against 0.124 ms at 192^3 (porosity 0.45)
-// compared with the contiguous order; 512 threads in the row-band order
-// gained 4 % at 256^3 but lost 4 % at 192^3.  The even step is best at 256.
+// Workgroup size of the odd (pull/push) half-step of the kinds that run at
+// full occupancy (FP64 flow, FP32 storage with the FP32 collision): 8-wave
+// workgroups keep more of its 19 gathers per node in flight on a CU (flow
+// FP64 at 256^3, porosity 0.40: 0.437 ms at 512 threads against 0.457 ms at
+// 256 and 0.491 ms at 1024, contiguous block order).  FP16 storage runs 256
+// threads in the row-band order with 256-node blocks (0.242 against 0.267 ms
+// at 256^3 in the contiguous order; 512 threads gained 4 % at 256^3 but lost
+// 4 % at 192^3).  StepShape below gives every kind's shape.
 constexpr int kBlockOdd = 512;
 // Launch shape of a half-step kernel instantiation: threads per workgroup,
 // and the number of its workgroups one CU may hold (0: as many as the

@@ -240,6 +240,38 @@ def test_row_band_block_order_fp16():
     g.close()
 
 
+@pytest.mark.parametrize("compute,tol", [(0, 1e-6), (1, 1e-5)])
+def test_row_band_block_order_fp32_storage(compute, tol):
+    """FP32 storage with the FP64 collision runs both half-steps in small
+    workgroups (one-wave odd blocks in the row-band map, two-wave even
+    blocks), with the FP32 collision in 512 / 256-thread ones: on a lattice
+    where the map is on, 24 steps from a random state stay within the
+    storage's error of the FP64 oracle (a skipped or repeated block would be
+    off by far more)"""
+    say(f"case row-band block order fp32 storage compute={compute}")
+    L = _pkg.load()
+    ni, nj, nk = 256, 160, 10
+    mask, _ = L.spheres3d(ni, nj, nk, 606, 3.0, 6.0, 0.45, zbuf=2)
+    nf = int((mask.reshape(-1) != 1).sum())
+    assert nf / nk / 8 >= 3 * 512, nf
+    kw = dict(dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(0.8, 0.75), zface_pressure=1, rho_bot=1.001,
+              rho_top=1.0, gaccel=(1e-6, -2e-6, 0, 0, 0, 0), zface_solute=1, c_bot=1.0, flux_plane=nk - 2)
+    o = Oracle(**kw)
+    o.set_solids(mask)
+    g = L.Lattice(mask=mask, storage=1, compute=compute, **kw)
+    f, gg = random_state(np.random.default_rng(71), 19, ni * nj * nk, 2)
+    load_both(o, g, f, gg)
+    o.run(24)
+    g.run(24)
+    fl = mask.reshape(-1) != 1
+    for s in range(2):
+        for c in range(4):
+            r = relnorm(g.mv(s)[c][fl], o.mv(s)[c][fl])
+            say(f"fp32 storage compute={compute} subs {s} field {c}: {r:.2e}")
+            assert r <= tol, (s, c, r)
+    g.close()
+
+
 @pytest.mark.parametrize("dims", [2, 3])
 def test_pest_partial_macro_write_and_upload(dims):
     """LBM_MACRO_PEST writes the flow velocity and the solute concentration
