@@ -314,7 +314,8 @@ __device__ __forceinline__ void scatter(const StepArgs<ST, CT> &A, int idx, cons
 template <typename ST>
 using pair_t = ST __attribute__((ext_vector_type(2)));
 
-template <int Q, typename ST, typename CT, bool ODD, bool OFF32>
+// NT: streaming loads also in the odd half-step (out-of-place steps)
+template <int Q, typename ST, typename CT, bool ODD, bool OFF32, bool NT = false>
 __device__ __forceinline__ void gather2(const StepArgs<ST, CT> &A, int idx, const int (&nb)[Q], CT (&f)[Q],
                                         CT (&g)[Q]) {
 #pragma unroll
@@ -326,7 +327,7 @@ __device__ __forceinline__ void gather2(const StepArgs<ST, CT> &A, int idx, cons
       off = nb[a] >= 0 ? foff<Q, 2>(0, a, nb[a]) : foff<Q, 2>(0, OPP(a), idx);
     else
       off = foff<Q, 2>(0, OPP(a), idx);
-    const pair_t<ST> v = ld_stream<!ODD>((const pair_t<ST> *)elem<OFF32>(A.F, off));
+    const pair_t<ST> v = ld_stream<!ODD || NT>((const pair_t<ST> *)elem<OFF32>(A.F, off));
     f[a] = Codec<ST, CT>::dec(v.x, A.inv_scale[0], (CT)0);
     g[a] = Codec<ST, CT>::dec(v.y, A.inv_scale[1], shift_of<Q, ST, CT, 1>(A, a));
   }
@@ -714,7 +715,17 @@ __global__ __launch_bounds__((half_step_block<ODD, NS, ST, CT>())) void k_half_s
   }
   if (NS == 2 && A.solute_on) {
     CT g[Q];
-    gather2<Q, ST, CT, ODD, OFF32>(A, idx, nb, f, g);
+    if constexpr (ODD && NS == 2 && sizeof(ST) == 8 && sizeof(CT) == 8) {
+      // An out-of-place odd step (the adjoint's forward replay, the kept
+      // steps of an objective run) does not write back the lines it reads:
+      // streaming loads leave the L2 to the lines it writes (128^3 flow +
+      // solute: replay 202 -> 188 ms per sweep, objective run with kept
+      // states 188 -> 183 ms; in-place steps unchanged).
+      if (A.Fo != A.F) gather2<Q, ST, CT, ODD, OFF32, true>(A, idx, nb, f, g);
+      else gather2<Q, ST, CT, ODD, OFF32>(A, idx, nb, f, g);
+    } else {
+      gather2<Q, ST, CT, ODD, OFF32>(A, idx, nb, f, g);
+    }
     if constexpr (Q == 19)
       if (zface_node<Q>(A, idx)) zface_fix<ST, CT, true>(A, idx, f, g);
     if constexpr (WALSH) {
