@@ -83,7 +83,7 @@ def run_workload(L, mask, kw, steps, warmup, label, ns=None, total_steps=0):
     return info, wall, prof, sums
 
 
-def adjoint_timing(L, n=128, steps=2000, window=200, every=100, keep_gb=250.0):
+def adjoint_timing(L, n=128, steps=2000, window=200, every=100, keep_gb=285.0):
     """discrete-adjoint workload: D3Q19 flow + solute 128^3, spheres to
     porosity 0.40 (seed 404), inlet map U[0.8, 1.2] (seed 4040); time of the
     objective (forward) run and of the adjoint sweep, replaying every segment

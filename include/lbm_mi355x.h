@@ -299,17 +299,20 @@ int lbm_objective_observations(lbm_handle *h, int n, const long *node, const int
  * Device memory: nsteps/every saved states (objective run) and every + 2
  * states (adjoint run), each the size of the population array; both stay
  * allocated for the next run and are released by lbm_set_solids and
- * lbm_destroy. */
+ * lbm_destroy.  An objective run may be swept more than once, except when
+ * it kept at least every + 1 states (lbm_objective_keep_states) and left
+ * segments to replay: the sweep then replays into the kept buffers, and a
+ * second sweep fails until the next objective run. */
 int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap);
 /* Device memory the next objective runs may spend on states kept for the
  * adjoint (no reference counterpart; default 0): the run keeps the states
  * of its last whole segments of `every` steps, as many as max_bytes holds
- * (one state is the size of the population array) while the checkpoints,
- * every + 2 states and 1 GiB of device memory stay free for the adjoint run,
- * and the adjoint
- * sweep uses them instead of replaying those segments from checkpoints.  The
- * gradient is the same bit for bit.  The kept states stay allocated like
- * the checkpoints. */
+ * (one state is the size of the population array) while the checkpoints of
+ * the other segments, the adjoint state, the states of one replayed segment
+ * (every + 1; the first kept buffers serve when there are that many) and
+ * 1 GiB of device memory stay free, and the adjoint sweep uses them instead
+ * of replaying those segments from checkpoints.  The gradient is the same
+ * bit for bit.  The kept states stay allocated like the checkpoints. */
 int lbm_objective_keep_states(lbm_handle *h, size_t max_bytes);
 
 /* Device-to-device copy of nbytes (reps times) on a stream of its own:

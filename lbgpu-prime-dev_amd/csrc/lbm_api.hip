@@ -132,6 +132,8 @@ struct lbm_handle {
     size_t keep_bytes = 0;     // lbm_objective_keep_states
     std::vector<void *> kept;  // states (after the BCs) before the last kept steps of the run
     int kept_from = 0;         // the run's first kept step index (0-based; nsteps: none)
+    int kept_replay = 0;       // the sweep replays into kept[0..every] (no segment buffers
+                               // of its own), which then no longer hold the run's states
     double *jacc = nullptr;
     double J = 0.0;
     double Jwin = 0.0;         // the window term of J
@@ -1532,20 +1534,37 @@ extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int ever
   // before them keep a checkpoint each
   long P = 0;
   if (h->obj.keep_bytes >= sb) {
-    long cap = (long)(h->obj.keep_bytes / sb);
-    // no more than leaves room for the checkpoints (at most one per segment),
-    // the adjoint's own buffers (every + 2 states) and 1 GiB besides, counting
-    // the kept states and checkpoints already held
+    const long cap = (long)(h->obj.keep_bytes / sb);
+    // The most steps (nsteps - m * every, m = 0, 1, ...) the budget holds
+    // whose states fit in device memory together with the checkpoints of the
+    // m replayed segments, the adjoint state and, when m > 0, the states of
+    // one replayed segment (every + 1, the first kept buffers when there are
+    // that many), and 1 GiB besides; memory these buffers already hold
+    // counts as available.
     size_t free_b = 0, total_b = 0;
     HIP_OK(hipMemGetInfo(&free_b, &total_b));
-    const double held = (double)(h->obj.kept.size() + h->obj.ckpt.size()) * (double)sb;
-    const double ckpt_max = (double)((nsteps + every - 1) / every) * (double)sb;
-    const double room = (double)free_b + held - ckpt_max - (double)(every + 2) * (double)sb - (double)(1ull << 30);
-    cap = std::min(cap, room > 0 ? (long)(room / (double)sb) : 0L);
-    P = cap >= nsteps ? nsteps : nsteps - (long)(nsteps - cap + every - 1) / every * every;
-    if (P < 0) P = 0;
+    const double held = (double)(h->obj.kept.size() + h->obj.ckpt.size() + h->obj.seg.size()) * (double)sb;
+    const double avail = (double)free_b + held - (double)(1ull << 30);
+    const long needL = h->obj.L ? 0 : 1;
+    for (long m = 0; m * every < nsteps; m++) {
+      const long p = nsteps - m * every;
+      const long segbuf = m > 0 && p < every + 1 ? every + 1 : 0;
+      if (p <= cap && (double)(p + m + segbuf + needL) * (double)sb <= avail) {
+        P = p;
+        break;
+      }
+    }
   }
   const int kept_from = nsteps - (int)P;
+  // With every + 1 kept states or more and segments to replay, the sweep
+  // replays them into the first kept buffers (it sweeps the kept segments
+  // first) instead of segment buffers of its own; the run then supports one
+  // sweep.
+  const bool kept_replay = kept_from > 0 && P >= every + 1;
+  if (kept_replay) {
+    for (void *q : h->obj.seg) (void)hipFree(q);
+    h->obj.seg.clear();
+  }
   while ((long)h->obj.kept.size() > P) {
     (void)hipFree(h->obj.kept.back());
     h->obj.kept.pop_back();
@@ -1657,6 +1676,7 @@ extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int ever
   }
   o.t0 = t0;
   o.kept_from = kept_from;
+  o.kept_replay = kept_replay ? 1 : 0;
   o.nsteps = nsteps;
   o.window = window;
   o.every = every;
@@ -1679,7 +1699,9 @@ extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int ever
 extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
   if (need_lattice(h) || adjoint_supported(h)) return 1;
   if (!grad) return lbm_fail("null gradient output");
-  if (!h->obj.valid) return lbm_fail("lbm_objective_run has not been called");
+  if (!h->obj.valid)
+    return lbm_fail("no objective run to differentiate: lbm_objective_run has not been called since the lattice, "
+                    "the objective or its observations were set, or since a sweep replayed into its kept states");
   const size_t sb = state_bytes(h);
   const int K = h->obj.every, T = h->obj.nsteps, W_ = h->obj.window, t0 = h->obj.t0;
   const int nseg = (T + K - 1) / K;
@@ -1695,10 +1717,12 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
     return rc;
   };
   hipError_t e;
-  // workspace: K + 1 segment states, the adjoint state and the gradient
+  // workspace: K + 1 segment states (or the first K + 1 kept buffers, free
+  // once the kept segments are swept), the adjoint state and the gradient
   // accumulators, kept for the next call
   std::vector<void *> &seg = h->obj.seg;
-  const size_t nsegbuf = nrep > 0 ? (size_t)K + 1 : 0;
+  const bool kept_replay = h->obj.kept_replay && nrep > 0 && h->obj.kept.size() >= (size_t)K + 1;
+  const size_t nsegbuf = nrep > 0 && !kept_replay ? (size_t)K + 1 : 0;
   while (seg.size() > nsegbuf) {
     (void)hipFree(seg.back());
     seg.pop_back();
@@ -1735,13 +1759,15 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
   const double seed = perm ? 0.0 : 1.0 / ((double)W_ * (double)h->E);
   const double kseed = perm && W_ > 0 ? nu / ((double)W_ * (double)h->N * perm_drive(h)) : 0.0;
   const bool off32 = !h->off64 && sb <= 0xffffffffull;
+  // where the replayed segments' states go
+  void *const *rp = kept_replay ? h->obj.kept.data() : seg.data();
   for (int sgi = nseg - 1; sgi >= 0 && !rc; sgi--) {
-    const int ts = t0 + sgi * K;                 // state seg[0] is before step ts+1
+    const int ts = t0 + sgi * K;                 // state rp[0] is before step ts+1
     const int len = std::min(K, t0 + T - ts);
     // the states before the segment's steps, after their BCs: kept by the
     // objective run, or replayed from the segment's checkpoint
-    void *const *sp = sgi >= nrep ? h->obj.kept.data() + ((long)sgi * K - h->obj.kept_from) : seg.data();
-    if (sgi < nrep && (e = hipMemcpyAsync(seg[0], h->obj.ckpt[sgi], sb, hipMemcpyDeviceToDevice, h->stream)) !=
+    void *const *sp = sgi >= nrep ? h->obj.kept.data() + ((long)sgi * K - h->obj.kept_from) : rp;
+    if (sgi < nrep && (e = hipMemcpyAsync(rp[0], h->obj.ckpt[sgi], sb, hipMemcpyDeviceToDevice, h->stream)) !=
                           hipSuccess) {
       fail("checkpoint restore", e);
       break;
@@ -1749,7 +1775,7 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
     // replay the segment out of place, keeping every state after its BCs
     for (int q = 1; q <= len && !rc && sgi < nrep; q++) {
       const int t = ts + q;
-      h->F = seg[q - 1];
+      h->F = rp[q - 1];
       const bool fz = zface_fused(h, t);
       if (!fz && launch_bcs(h, (t & 1) ? 1 : 2, t)) {
         rc = 1;
@@ -1757,7 +1783,7 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
       }
       StepArgs<double, double> A = make_args<double, double>(h);
       if (fz) set_zb(h, A);
-      A.Fo = (double *)seg[q];
+      A.Fo = (double *)rp[q];
       A.solute_on = 1;
       const int bs = (t & 1) ? half_step_block<true, 2, double, double>() : half_step_block<false, 2, double, double>();
       const size_t lds = (t & 1) ? step_pad<true, 2, double, double>(h) : step_pad<false, 2, double, double>(h);
@@ -1845,6 +1871,9 @@ extern "C" int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap) {
     }
   }
   h->F = saved_F;
+  // the first kept buffers now hold replayed states: this run's kept
+  // segments are gone, and another sweep needs another objective run
+  if (kept_replay) h->obj.valid = 0;
   if (!rc) {
     std::vector<double> part((size_t)nslot * 5);
     if ((e = hipMemcpyAsync(part.data(), gacc, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream)) !=

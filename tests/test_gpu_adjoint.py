@@ -223,7 +223,8 @@ def test_permeability_adjoint_matches_tlm_and_differences():
 
 
 @pytest.mark.parametrize("case", ["half_kept", "all_kept", "short_last_segment_kept", "none_kept",
-                                  "half_kept_permeability_observations"])
+                                  "half_kept_permeability_observations",
+                                  "two_thirds_kept_permeability_observations"])
 def test_kept_states_give_the_same_gradient(case):
     """lbm_objective_keep_states: the objective run keeps the states of its
     last whole segments and the sweep uses them instead of replaying; J, the
@@ -233,11 +234,16 @@ def test_kept_states_give_the_same_gradient(case):
     L, mask, cmap = setup()
     # (nsteps, window, every, states the budget holds): 40 of 80 steps kept;
     # all 70 (the last of 3 segments has 20 steps); only the last segment of
-    # 10 steps (60 = 2 x 25 + 10); none (9 states hold no whole segment)
+    # 10 steps (60 = 2 x 25 + 10); none (9 states hold no whole segment);
+    # 20 of 60; 40 of 60.  With at least every + 1 states kept and segments
+    # left to replay (40 of 80, 40 of 60) the sweep replays them into the
+    # first kept buffers.
     nsteps, window, every, nkeep = {"half_kept": (80, 20, 20, 40), "all_kept": (70, 15, 25, 1000),
                                     "short_last_segment_kept": (60, 20, 25, 10),
                                     "none_kept": (60, 20, 25, 9),
-                                    "half_kept_permeability_observations": (60, 20, 20, 30)}[case]
+                                    "half_kept_permeability_observations": (60, 20, 20, 30),
+                                    "two_thirds_kept_permeability_observations": (60, 20, 20, 40)}[case]
+    kept_replay = case in ("half_kept", "two_thirds_kept_permeability_observations")
     # the permeability window writes the velocity on its steps, the observed
     # steps the concentration (in kept and in replayed segments)
     kind = "permeability" if "permeability" in case else "flux"
@@ -259,6 +265,13 @@ def test_kept_states_give_the_same_gradient(case):
         grad, gmap = lat.adjoint()
         f_after = lat.download_f(0), lat.download_f(1)
         out.append((J, grad, gmap, f_after))
+        if keep and kept_replay:
+            # the kept buffers now hold replayed states: no second sweep
+            with pytest.raises(L.LbmError, match="since a sweep replayed into its kept states"):
+                lat.adjoint()
+        else:
+            grad2, gmap2 = lat.adjoint()
+            assert np.array_equal(grad2, grad) and np.array_equal(gmap2, gmap)
         lat.close()
     (J0, g0, m0, f0), (J1, g1, m1, f1) = out
     say(f"J {J0:.15e} / {J1:.15e}; grad {g0} / {g1}")
