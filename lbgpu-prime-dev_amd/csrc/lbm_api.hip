@@ -1556,12 +1556,15 @@ extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int ever
     }
   }
   const int kept_from = nsteps - (int)P;
+  const size_t nck = (size_t)((kept_from + every - 1) / every);  // kept_from: nsteps or a multiple of every
   // With every + 1 kept states or more and segments to replay, the sweep
   // replays them into the first kept buffers (it sweeps the kept segments
   // first) instead of segment buffers of its own; the run then supports one
   // sweep.
-  const bool kept_replay = kept_from > 0 && P >= every + 1;
-  if (kept_replay) {
+  const bool kept_replay = nck > 0 && P >= every + 1;
+  // release what this run and its sweep will not use before allocating:
+  // the sizing above counted it as available
+  if (nck == 0 || kept_replay) {
     for (void *q : h->obj.seg) (void)hipFree(q);
     h->obj.seg.clear();
   }
@@ -1569,15 +1572,14 @@ extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int ever
     (void)hipFree(h->obj.kept.back());
     h->obj.kept.pop_back();
   }
+  while (h->obj.ckpt.size() > nck) {
+    (void)hipFree(h->obj.ckpt.back());
+    h->obj.ckpt.pop_back();
+  }
   while ((long)h->obj.kept.size() < P) {
     void *c = nullptr;
     HIP_OK(hipMalloc(&c, sb));
     h->obj.kept.push_back(c);
-  }
-  const size_t nck = (size_t)((kept_from + every - 1) / every);  // kept_from: nsteps or a multiple of every
-  while (h->obj.ckpt.size() > nck) {
-    (void)hipFree(h->obj.ckpt.back());
-    h->obj.ckpt.pop_back();
   }
   while (h->obj.ckpt.size() < nck) {
     void *c = nullptr;
