@@ -133,6 +133,7 @@ def adjoint_timing(L, n=128, steps=2000, window=200, every=100, keep_gb=285.0):
         t0 = time.perf_counter()
         J_k = lat.objective(steps, window, every)
         t_fwd_k = time.perf_counter() - t0
+        kept["kept_steps"] = lat.objective_info()["kept_steps"]
         t0 = time.perf_counter()
         grad_k, gmap_k = lat.adjoint()
         t_adj_k = time.perf_counter() - t0

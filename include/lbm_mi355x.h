@@ -314,6 +314,11 @@ int lbm_adjoint_run(lbm_handle *h, double *grad, double *grad_cmap);
  * of replaying those segments from checkpoints.  The gradient is the same
  * bit for bit.  The kept states stay allocated like the checkpoints. */
 int lbm_objective_keep_states(lbm_handle *h, size_t max_bytes);
+/* What the last objective run holds for its adjoint (no reference
+ * counterpart): the number of its last steps whose states it kept
+ * (lbm_objective_keep_states) and the number of segment checkpoints it
+ * saved; both 0 before any objective run. */
+int lbm_objective_info(const lbm_handle *h, int *kept_steps, int *checkpoints);
 
 /* Device-to-device copy of nbytes (reps times) on a stream of its own:
  * the measured bandwidth in GB/s (bytes read + written per second) is the

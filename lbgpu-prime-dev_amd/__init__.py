@@ -109,6 +109,7 @@ def lib():
             "lbm_set_ns": [vp, dp],
             "lbm_objective_select": [vp, ip],
             "lbm_objective_keep_states": [vp, ctypes.c_size_t],
+            "lbm_objective_info": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
             "lbm_sample_macro": [vp, ip, ip, ctypes.POINTER(ctypes.c_long), dp],
             "lbm_reduce": [vp, ip, ip, dp],
             "lbm_last_flux": [vp, dp, dp],
@@ -400,6 +401,12 @@ class Lattice:
         """device memory the next objective runs may spend on states kept
         for the adjoint (lbm_objective_keep_states)"""
         _check(lib().lbm_objective_keep_states(self.h, int(max_bytes)))
+
+    def objective_info(self):
+        """{"kept_steps", "checkpoints"} of the last objective run"""
+        kept, ck = ctypes.c_int(), ctypes.c_int()
+        _check(lib().lbm_objective_info(self.h, ctypes.byref(kept), ctypes.byref(ck)))
+        return {"kept_steps": kept.value, "checkpoints": ck.value}
 
     def adjoint(self, with_map=True):
         """(dJ/d(gx, gy, gz, tau, tau_s), dJ/dc_bot_map or None)"""

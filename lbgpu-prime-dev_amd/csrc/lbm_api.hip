@@ -1514,6 +1514,13 @@ extern "C" int lbm_objective_keep_states(lbm_handle *h, size_t max_bytes) {
   return 0;
 }
 
+extern "C" int lbm_objective_info(const lbm_handle *h, int *kept_steps, int *checkpoints) {
+  if (!h || !kept_steps || !checkpoints) return lbm_fail("null argument");
+  *kept_steps = (int)h->obj.kept.size();
+  *checkpoints = (int)h->obj.ckpt.size();
+  return 0;
+}
+
 extern "C" int lbm_objective_run(lbm_handle *h, int nsteps, int window, int every, double *J) {
   if (need_lattice(h) || adjoint_supported(h)) return 1;
   const int nobs = (int)h->obj.o_step.size();

@@ -244,6 +244,10 @@ def test_kept_states_give_the_same_gradient(case):
                                     "half_kept_permeability_observations": (60, 20, 20, 30),
                                     "two_thirds_kept_permeability_observations": (60, 20, 20, 40)}[case]
     kept_replay = case in ("half_kept", "two_thirds_kept_permeability_observations")
+    # steps kept and checkpoints saved: the last whole segments the budget holds
+    kept_steps = {"half_kept": 40, "all_kept": 70, "short_last_segment_kept": 10, "none_kept": 0,
+                  "half_kept_permeability_observations": 20,
+                  "two_thirds_kept_permeability_observations": 40}[case]
     # the permeability window writes the velocity on its steps, the observed
     # steps the concentration (in kept and in replayed segments)
     kind = "permeability" if "permeability" in case else "flux"
@@ -262,6 +266,9 @@ def test_kept_states_give_the_same_gradient(case):
             lat.observations(*observations(mask))
         lat.init_rest()
         J = lat.objective(nsteps, window, every, kind=kind)
+        info = lat.objective_info()
+        k = kept_steps if keep else 0
+        assert info == {"kept_steps": k, "checkpoints": (nsteps - k + every - 1) // every}, info
         grad, gmap = lat.adjoint()
         f_after = lat.download_f(0), lat.download_f(1)
         out.append((J, grad, gmap, f_after))
