@@ -83,6 +83,10 @@ def run_workload(L, mask, kw, steps, warmup, label, ns=None, total_steps=0):
     return info, wall, prof, sums
 
 
+class KeepSkipped(Exception):
+    """the kept-state part of the adjoint workload was not requested"""
+
+
 def adjoint_timing(L, n=128, steps=2000, window=200, every=100, keep_gb=285.0):
     """discrete-adjoint workload: D3Q19 flow + solute 128^3, spheres to
     porosity 0.40 (seed 404), inlet map U[0.8, 1.2] (seed 4040); time of the
@@ -125,6 +129,8 @@ def adjoint_timing(L, n=128, steps=2000, window=200, every=100, keep_gb=285.0):
     # run (lbm_objective_keep_states): those segments are not replayed
     kept = {"budget_gb": keep_gb}
     try:
+        if keep_gb <= 0:
+            raise KeepSkipped()
         lat.keep_states(int(keep_gb * 1e9))
         start()
         lat.objective(steps, window, every)
@@ -144,6 +150,8 @@ def adjoint_timing(L, n=128, steps=2000, window=200, every=100, keep_gb=285.0):
                      "same_gradient_bits": same})
         log(f"[adjoint] kept states ({keep_gb:.0f} GB): forward {t_fwd_k:.3f}s adjoint {t_adj_k:.3f}s, "
             f"same gradient {same}")
+    except KeepSkipped:
+        kept["skipped"] = "no kept-state budget (--keep-gb 0)"
     except L.LbmError as err:
         # device memory the budget assumed was not there: reported, not fatal
         kept["error"] = str(err)
@@ -429,6 +437,8 @@ def main():
                     help="of the extras, run only the 256^3 flow + solute storage and Walsh variants")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="of the extras, run only the 128^3 discrete-adjoint workload")
+    ap.add_argument("--keep-gb", type=float, default=285.0,
+                    help="device memory (GB) the adjoint workload's objective run may keep states in; 0: none")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "the benchmark runs on one GPU; --gpus above 1 is not supported"}))
@@ -517,7 +527,7 @@ def main():
     }
 
     if args.adjoint_only:
-        ext["adjoint_c4"] = adjoint_timing(L)
+        ext["adjoint_c4"] = adjoint_timing(L, keep_gb=args.keep_gb)
     elif not args.no_extra:
         variants = [("flow_solute_fp32_shifted", 1, 1, 4), ("flow_solute_fp32_shifted_f64_collision", 1, 0, 4),
                     ("flow_solute_fp16_scaled", 2, 1, 2)]
@@ -555,7 +565,7 @@ def main():
         if "flow_solute_fp32_shifted" in ext:
             ext["fp32_over_fp64_speed"] = round(ext["flow_solute_fp32_shifted"]["mflups"] / mflups, 3)
         if not args.variants_only:
-            ext["adjoint_c4"] = adjoint_timing(L)
+            ext["adjoint_c4"] = adjoint_timing(L, keep_gb=args.keep_gb)
             ext["throughput_c3"] = throughput_c3(L, cpu_s=0.0 if args.no_cpu else 9.0)
             ext["throughput_c1"] = throughput_c1(L, cpu=not args.no_cpu)
             ext["precision_c3"] = precision_table(L)
