@@ -1,6 +1,7 @@
 """CPU-side checks of the product library: it loads without a GPU, exports
 every entry point the C header declares, and its seeded geometry generator
 is reproducible and meets its stated properties."""
+import ctypes
 import hashlib
 import os
 import re
@@ -153,3 +154,15 @@ def test_create_rejects_bad_parameters(bad, match):
     kw.update(bad)
     with pytest.raises(L.LbmError, match=match):
         L.Lattice(**kw)
+
+
+def test_handle_calls_reject_null_arguments():
+    """calls on a handle check their pointers before anything else (no GPU
+    needed): a null handle or output is an error with its reason"""
+    L = _pkg.load()
+    lib = L.lib()
+    kept, ck = ctypes.c_int(-1), ctypes.c_int(-1)
+    assert lib.lbm_objective_info(None, ctypes.byref(kept), ctypes.byref(ck)) != 0
+    assert b"null argument" in lib.lbm_last_error()
+    assert lib.lbm_objective_keep_states(None, 1) != 0
+    assert b"null handle" in lib.lbm_last_error()
