@@ -98,6 +98,13 @@ typedef struct lbm_params {
   int zface_conc[2];   /* enum lbm_zface_conc (solute) */
   double c_top;        /* concentration of a LBM_ZCONC_DIRICHLET z = nk-1 face
                           (the z = 0 face takes c_bot and c_bot_map) */
+  /* halo_mode = 1, phase-2 unpack: 0 (default) stores a received value only
+   * where the pushing node across the face is fluid, so the exchanged run
+   * equals the periodic one; 1 stores every position of the received planes
+   * as k_bound_HtD_2 does (src/boundary_kernels.c:106-139), which replaces
+   * the own-slot bounce-back value of a face node whose cross-face upstream
+   * node is solid with the received (stale) ghost value. */
+  int halo_literal;
 } lbm_params;
 
 /* flow condition of a z face: pressure takes rho_bot / rho_top, velocity

@@ -69,6 +69,7 @@ class Params(ctypes.Structure):
         ("uy_in", ctypes.c_double), ("uy_out", ctypes.c_double),
         ("zface_flow", ctypes.c_int * 2), ("zface_uz", ctypes.c_double * 2),
         ("zface_conc", ctypes.c_int * 2), ("c_top", ctypes.c_double),
+        ("halo_literal", ctypes.c_int),
     ]
 
 
@@ -189,6 +190,7 @@ def make_params(**kw):
         arr = getattr(p, name)
         arr[0], arr[1] = v[0], v[1]
     p.c_top = kw.get("c_top", 0.0)
+    p.halo_literal = kw.get("halo_literal", 0)
     return p
 
 

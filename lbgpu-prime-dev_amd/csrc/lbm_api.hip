@@ -623,6 +623,7 @@ HaloMaps halo_maps(const lbm_handle *h) {
   H.pairs = (h->Q - h->U) / 2;
   H.U = h->U;
   H.ghost = h->nf;
+  H.literal = h->p.halo_literal;
   return H;
 }
 
@@ -697,6 +698,7 @@ extern "C" int lbm_create(const lbm_params *p, lbm_handle **out) {
   if (p->dims != 2 && p->dims != 3) return lbm_fail("dims must be 2 or 3");
   if (p->ni <= 0 || p->nj <= 0 || (p->dims == 3 && p->nk <= 0)) return lbm_fail("bad lattice size");
   if (p->nsubs != 1 && p->nsubs != 2) return lbm_fail("nsubs must be 1 or 2");
+  if (p->halo_literal != 0 && p->halo_literal != 1) return lbm_fail("halo_literal must be 0 or 1");
   for (int s = 0; s < p->nsubs; s++)
     if (!(p->tau[s] > 0.5)) return lbm_fail("tau[%d] = %g must exceed 1/2", s, p->tau[s]);
   if (p->storage < 0 || p->storage > 2 || p->compute < 0 || p->compute > 1)

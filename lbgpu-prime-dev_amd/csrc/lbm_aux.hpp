@@ -74,6 +74,7 @@ struct HaloMaps {
   const int *tgt_lo;    // [pairs][E] phase-2 target in the first plane or -1
   const int *tgt_hi;    // [pairs][E] phase-2 target in the last plane or -1
   int E, pairs, U, ghost;
+  int literal;          // phase 2 stores every plane position (k_bound_HtD_2)
 };
 
 template <int Q, int NS, typename ST, typename CT>
@@ -109,7 +110,9 @@ __global__ void k_halo_unpack(StepArgs<ST, CT> A, HaloMaps H, int nsubs, int pha
     A.F[foff<Q, NS>(s, a, H.ghost + m)] = pos[t];
     A.F[foff<Q, NS>(s, a + 1, H.ghost + H.E + m)] = neg[t];
   } else {
-    int hi = H.tgt_hi[(long)q * H.E + m], lo = H.tgt_lo[(long)q * H.E + m];
+    // the masked targets, or every fluid node of the two planes
+    const int hi = H.literal ? H.plane_hi[m] : H.tgt_hi[(long)q * H.E + m];
+    const int lo = H.literal ? H.plane_lo[m] : H.tgt_lo[(long)q * H.E + m];
     if (hi >= 0) A.F[foff<Q, NS>(s, a, hi)] = neg[t];
     if (lo >= 0) A.F[foff<Q, NS>(s, a + 1, lo)] = pos[t];
   }

@@ -741,10 +741,11 @@ void orc_halo_pack(const orc *o, int phase, double *pos, double *neg) {
 
 /* Phase 2 stores a received value only where the neighbour's push
  * actually happened, i.e. where the pushing node across the face is fluid.
- * k_bound_HtD_2 stores every position, which overwrites the own-slot
- * bounce-back value of a face node whose cross-face upstream node is solid
- * with a stale copy; the masked form keeps the exchanged run identical to
- * the periodic one. */
+ * k_bound_HtD_2 stores every position (src/boundary_kernels.c:106-139),
+ * which overwrites the own-slot bounce-back value of a face node whose
+ * cross-face upstream node is solid with a stale copy; the masked form
+ * keeps the exchanged run identical to the periodic one.  halo_literal
+ * selects the reference's form. */
 void orc_halo_unpack(orc *o, int phase, const double *pos, const double *neg) {
   long E = o->E, N = o->N;
   int nb = (o->Q - o->U) / 2;
@@ -761,8 +762,9 @@ void orc_halo_unpack(orc *o, int phase, const double *pos, const double *neg) {
           int i = (int)(m % ni), j = (int)(m / ni);
           int iu = (i + VX[a] + ni) % ni, ju = (j + VY[a] + nj) % nj;
           int id = (i - VX[a] + ni) % ni, jd = (j - VY[a] + nj) % nj;
-          if (o->sol[o->E + N + iu + (long)ni * ju] != 1) fslot(o, s, a)[N - E + m] = ne[m];
-          if (o->sol[id + (long)ni * jd] != 1) fslot(o, s, a + 1)[m] = po[m];
+          const int lit = o->p.halo_literal;
+          if (lit || o->sol[o->E + N + iu + (long)ni * ju] != 1) fslot(o, s, a)[N - E + m] = ne[m];
+          if (lit || o->sol[id + (long)ni * jd] != 1) fslot(o, s, a + 1)[m] = po[m];
         }
       }
     }

@@ -84,6 +84,10 @@ typedef struct orc_params {
   double zface_uz[2];
   int zface_conc[2];
   double c_top;
+  /* halo_mode 1, phase-2 unpack: 0 stores only where the pushing node
+   * across the face is fluid; 1 stores every position (k_bound_HtD_2,
+   * src/boundary_kernels.c:106-139) */
+  int halo_literal;
 } orc_params;
 
 typedef struct orc orc;

@@ -415,6 +415,32 @@ def test_halo_pack_buffers_match_oracle():
     compare(o, g, mask, 2, "explicit halo buffers, 2 steps")
 
 
+def test_halo_literal_unpack_matches_oracle():
+    """halo_literal = 1 (the phase-2 unpack of k_bound_HtD_2, storing every
+    received position) on a random lattice with solids on both slab faces:
+    10 steps of the GPU path equal the oracle's literal mode bit for bit, and
+    they differ from the default (masked) exchange"""
+    say("case halo literal unpack")
+    ni, nj, nk = 10, 8, 12
+    rng = np.random.default_rng(21)
+    mask = (rng.random((nk, nj, ni)) < 0.3).astype(np.uint8)
+    f, gg = random_state(rng, 19, mask.size, 2)
+    runs = {}
+    for lit in (1, 0):
+        kw = dict(dims=3, ni=ni, nj=nj, nk=nk, nsubs=2, tau=(0.8, 0.7), halo_mode=1, halo_literal=lit,
+                  gaccel=(1e-5, 0, 2e-5, 0, 0, 0))
+        o, g = build_pair(mask, **kw)
+        load_both(o, g, f, gg)
+        o.run(10)
+        g.run(10)
+        _, bitwise = compare(o, g, mask, 2, f"halo literal={lit}, 10 steps")
+        assert bitwise
+        runs[lit] = g.download_f(0)
+        g.close()
+    fl = mask.reshape(-1) != 1
+    assert not np.array_equal(runs[1][:, fl], runs[0][:, fl])
+
+
 @pytest.mark.parametrize("storage,compute,tol", [(0, 1, 1e-5), (1, 1, 1e-5), (1, 0, 1e-6), (2, 1, 1e-2)])
 def test_reduced_precision_error_vs_fp64(storage, compute, tol):
     """storage FP64/FP32/FP16 with FP32 collision against the FP64 oracle

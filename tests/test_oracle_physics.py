@@ -309,6 +309,55 @@ def test_halo_self_exchange_equals_wrap(dims, shape, nsubs):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("dims,shape", [(3, (6, 5, 4)), (2, (7, 6))])
+def test_halo_literal_unpack_rule(dims, shape):
+    """halo_literal (the phase-2 unpack of k_bound_HtD_2, which stores every
+    received position) changes a run only through face nodes whose
+    cross-face upstream node is solid: with fluid face planes it equals the
+    masked exchange bit for bit over 12 steps; with solids on the faces, one
+    phase-2 unpack after the same odd step leaves states that differ, and
+    only in the two face planes"""
+    rng = np.random.default_rng(41)
+    Q = 9 if dims == 2 else 19
+    N = int(np.prod(shape))
+    E = N // shape[0]  # nodes per plane across the last axis
+    f = rng.uniform(-1e-3, 1e-3, (Q, N))
+    g = rng.uniform(0, 0.1, (Q, N))
+    kw = dict(dims=dims, ni=shape[-1], nj=shape[-2], nk=shape[0] if dims == 3 else 1, nsubs=2,
+              tau=(0.8, 0.7), gaccel=(1e-4, 2e-4, -1e-4, 0, 0, 0), halo_mode=1)
+
+    def oracle(mask, lit):
+        o = Oracle(halo_literal=lit, **kw)
+        o.set_solids(mask)
+        o.set_f(0, f)
+        o.set_f(1, g)
+        return o
+
+    open_faces = (rng.random(shape) < 0.3).astype(np.uint8)
+    open_faces[0] = 0
+    open_faces[-1] = 0
+    res = []
+    for lit in (0, 1):
+        o = oracle(open_faces, lit)
+        o.run(12)
+        res.append([o.get_f(0), o.get_f(1), o.mv(0)[0]])
+    for a, b in zip(*res):
+        assert np.array_equal(a, b)
+
+    solid_faces = (rng.random(shape) < 0.35).astype(np.uint8)
+    assert solid_faces[0].any() and solid_faces[-1].any()
+    states = []
+    for lit in (0, 1):
+        o = oracle(solid_faces, lit)
+        o.step()
+        pos, neg = o.halo_pack(2)
+        o.halo_unpack(2, pos, neg)
+        states.append(np.concatenate([o.get_f(0), o.get_f(1)]))
+    diff = np.any(states[0] != states[1], axis=0)
+    assert diff.any()
+    assert not diff[E:N - E].any()
+
+
 def test_reductions_brute_force():
     rng = np.random.default_rng(8)
     shape = (6, 7, 5)
