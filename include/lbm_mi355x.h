@@ -324,7 +324,9 @@ int lbm_objective_keep_states(lbm_handle *h, size_t max_bytes);
 /* What the last objective run holds for its adjoint (no reference
  * counterpart): the number of its last steps whose states it kept
  * (lbm_objective_keep_states) and the number of segment checkpoints it
- * saved; both 0 before any objective run. */
+ * saved; both 0 before any objective run.  The counts describe the run's
+ * buffers: after a sweep that replayed into the kept buffers they no longer
+ * hold the run's states (lbm_adjoint_run then refuses a second sweep). */
 int lbm_objective_info(const lbm_handle *h, int *kept_steps, int *checkpoints);
 
 /* Device-to-device copy of nbytes (reps times) on a stream of its own:
