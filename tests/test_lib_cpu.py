@@ -145,6 +145,7 @@ def test_benchmark_geometries_recorded(name, args, sha, phi):
     (dict(ni=40000, nj=2), "LX = 40000 above 32767 is not supported"),
     (dict(dims=3, nk=5, flux_plane=5), "flux_plane 5 lies outside"),
     (dict(flux_plane=6), "flux_plane 6 lies outside"),
+    (dict(halo_literal=2), "halo_literal must be 0 or 1"),
 ])
 def test_create_rejects_bad_parameters(bad, match):
     """lbm_create checks its parameters before it touches the GPU, and
